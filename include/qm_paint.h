@@ -1,0 +1,81 @@
+/* libqmpaint: the MI355X noise painter's C ABI.
+ *
+ * The drop-in boundary for the reference's paint-gpu crate (paint-gpu/src/lib.rs:14-22): the
+ * Rust side keeps reading tiles and calls these functions through `extern "C"`; the library owns
+ * everything on the device. Every call returns 0 on success, or a nonzero status with a message
+ * in `err` (at most `errlen` bytes, NUL-terminated) that the Rust wrapper maps to Err(String).
+ * Per-pair soft failures are not errors: they come back as QM_FAILED_* bits per pair (as
+ * paint-gpu/kernels/common.cuh:27-31), for the caller's CPU fallback (paint-gpu/src/batch.rs:292-317).
+ *
+ * Ownership: input arrays are borrowed for the duration of the call; handles are freed by their
+ * *_free function. Threading: a qm_gpu may be used from several threads; each call uses its own
+ * stream and buffers. Scene layout: qm_scene.h (in quietmap_amd/csrc/).
+ */
+#ifndef QM_PAINT_H
+#define QM_PAINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../quietmap_amd/csrc/qm_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qm_gpu qm_gpu;
+typedef struct qm_square qm_square;
+
+/* Gpu::new(ordinal) (paint-gpu/src/device.rs:254-420): opens the device and checks that it is a
+ * gfx950 with the library's kernels loaded. */
+int qm_gpu_new(int ordinal, qm_gpu** out, char* err, size_t errlen);
+/* Gpu::name (paint-gpu/src/device.rs:422-425). */
+int qm_gpu_name(const qm_gpu* gpu, char* out, size_t len);
+void qm_gpu_free(qm_gpu* gpu);
+
+/* DeviceSquare::upload (paint-gpu/src/square_upload.rs:18-287): the scene's terrain lattice,
+ * obstacle cell index, weather lattice and point sources (the candidates with their attributes),
+ * copied to the device. The sources are bucketed on a grid of `bucket_m` metres for the painter. */
+int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles* obstacles,
+                     const qm_weather* weather, const qm_source* sources, size_t n_sources, double bucket_m,
+                     qm_square** out, char* err, size_t errlen);
+/* DeviceSquare::bytes (paint-gpu/src/square_upload.rs:283-287): device bytes the square holds. */
+size_t qm_square_bytes(const qm_square* square);
+void qm_square_free(qm_square* square);
+
+/* Gpu::evaluate_pairs (paint-gpu/src/device.rs:440-765): for every pair (point, source) the
+ * received energy per period mixed by the weather, then calm and downwind alone
+ * (`sums`, 9 floats a pair: mixed[3], calm[3], favourable[3]; paint/src/exact.rs:129-133), its
+ * failure bits, and every point's bands per layer, period and band (`point_bands`, optional,
+ * n_points*6*3*8 floats, summed with the optional per-pair `pair_weight`; exact.rs:208-216).
+ * Host arrays in and out. */
+int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
+                      const uint32_t* pair_point, const uint32_t* pair_source, const float* pair_weight,
+                      size_t n_pairs, float* sums, uint32_t* failed, float* point_bands, char* err, size_t errlen);
+
+/* The painted grid: receivers on a row-major grid (`points`, nx*ny, x fastest), every source
+ * within `cutoff_m` (horizontal) of each, in one persistent work-stealing launch. Out per
+ * receiver: the summed mixed energy per period (`period_energy`, 3 doubles; Lden follows from
+ * physics/src/bands.rs:59-66), the paths evaluated and the paths that failed. Host arrays. */
+int qm_paint_grid(qm_gpu* gpu, const qm_square* square, const qm_point* points, uint32_t nx, uint32_t ny,
+                  double cutoff_m, double* period_energy, uint32_t* paths, uint32_t* failures, char* err,
+                  size_t errlen);
+
+/* The same with device arrays (already resident in HBM) on a caller's stream (hipStream_t, or
+ * NULL for the default stream); returns after enqueueing. */
+int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, uint32_t nx, uint32_t ny,
+                         double cutoff_m, double* d_period_energy, uint32_t* d_paths, uint32_t* d_failures,
+                         void* stream, char* err, size_t errlen);
+
+/* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
+ * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings
+ * (t, footprint, outline, edge index, flags bit 0 building / bit 1 bounds roof) and the count. */
+int qm_crossings(qm_gpu* gpu, const qm_square* square, const double* segments, size_t n_segments,
+                 uint32_t capacity, double* t, uint64_t* footprint, uint32_t* outline, uint32_t* edge,
+                 uint32_t* flags, uint32_t* count, uint32_t* failed, char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

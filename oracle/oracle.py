@@ -1,0 +1,226 @@
+"""TEST INFRASTRUCTURE — ctypes binding of the FP64 CPU oracle (``liboracle.so``).
+
+Only the tests, ``__graft_entry__.smoke()`` and the bench's CPU-baseline leg import this module,
+and only to check or time against it; the product path (``quietmap_amd``) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from quietmap_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            build()
+        _lib = C.CDLL(LIBRARY)
+        _declare(_lib)
+    return _lib
+
+
+D = C.POINTER(C.c_double)
+U8 = C.POINTER(C.c_uint8)
+U32 = C.POINTER(C.c_uint32)
+U64 = C.POINTER(C.c_uint64)
+F32 = C.POINTER(C.c_float)
+SZ = C.POINTER(C.c_size_t)
+
+
+def _declare(L):
+    L.qmo_alpha_bands.argtypes = [C.c_double, C.c_double, D]
+    L.qmo_lden_energy.argtypes = [D]
+    L.qmo_lden_energy.restype = C.c_double
+    L.qmo_state_boundaries.argtypes = [C.c_size_t, D, D, D, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_size_t, D, D, C.c_size_t, D, D, D, D]
+    L.qmo_ray_terms.argtypes = [C.c_size_t, C.c_double, D, D, D, D, D, C.c_size_t, D, D, U8, U8, U64,
+                                C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                C.c_uint64, D, D, D, D, D, D]
+    L.qmo_vertical_path.argtypes = [C.c_size_t, C.c_double, D, D, D, C.c_size_t, D, D, U8, U8, U64, C.c_double,
+                                    C.c_double, C.c_uint64, C.c_size_t, D, D, D, SZ, D, SZ, D, SZ]
+    L.qmo_ground_at.argtypes = [C.c_void_p, C.c_double, C.c_double, D]
+    L.qmo_fill_profile.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_size_t,
+                                   D, D, D, D, D, SZ, D]
+    L.qmo_crossings.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_size_t,
+                                D, D, U8, U8, U64, U32, U32, SZ, C.c_int]
+    L.qmo_receivers.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, D, D, D, U32]
+    L.qmo_evaluate_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                     C.c_size_t, U32, U32, F32, C.c_size_t, D, U32, D, C.c_int, U64]
+    L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                   C.c_size_t, C.c_double, D, U64, U32, C.c_int]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def alpha_bands(temperature_c, relative_humidity_pct):
+    out = np.zeros(8)
+    lib().qmo_alpha_bands(temperature_c, relative_humidity_pct, _p(out, D))
+    return out
+
+
+def lden_energy(period_energy):
+    e = np.ascontiguousarray(period_energy, np.float64)
+    return lib().qmo_lden_energy(_p(e, D))
+
+
+def state_boundaries(distance, altitude, ground, source, receiver, source_ground_factor, terrain, tops):
+    d, z, g = (np.ascontiguousarray(a, np.float64) for a in (distance, altitude, ground))
+    terrain = np.ascontiguousarray(np.asarray(terrain, np.float64).reshape(-1, 2))
+    tops = np.ascontiguousarray(np.asarray(tops, np.float64).reshape(-1, 2))
+    tx, tz = np.ascontiguousarray(terrain[:, 0]), np.ascontiguousarray(terrain[:, 1])
+    ox, oz = np.ascontiguousarray(tops[:, 0]), np.ascontiguousarray(tops[:, 1])
+    att, wo = np.zeros(16), np.zeros(16)
+    lib().qmo_state_boundaries(d.size, _p(d, D), _p(z, D), _p(g, D), source[0], source[1], receiver[0], receiver[1],
+                               source_ground_factor, tx.size, _p(tx, D), _p(tz, D), ox.size, _p(ox, D), _p(oz, D),
+                               _p(att, D), _p(wo, D))
+    return att.reshape(2, 8), wo.reshape(2, 8)
+
+
+def ray_terms(profile, crossings, ends, favourable, alpha):
+    """profile: dict(horizontal_m, t, ground_m, ground_factor, forest_cover, canopy_m);
+    crossings: list of (t, height_m, building, bounds_roof, footprint_id); ends: dict."""
+    arr = {k: np.ascontiguousarray(profile[k], np.float64) for k in ("t", "ground_m", "ground_factor")}
+    n = arr["t"].size
+    forest = np.ascontiguousarray(profile.get("forest_cover", np.zeros(n)), np.float64)
+    canopy = np.ascontiguousarray(profile.get("canopy_m", np.zeros(n)), np.float64)
+    ct = np.array([c[0] for c in crossings], np.float64)
+    ch = np.array([c[1] for c in crossings], np.float64)
+    cb = np.array([bool(c[2]) for c in crossings], np.uint8)
+    cr = np.array([bool(c[3]) for c in crossings], np.uint8)
+    cf = np.array([c[4] for c in crossings], np.uint64)
+    fav = np.ascontiguousarray(favourable, np.float64)
+    al = np.ascontiguousarray(alpha, np.float64)
+    periods, states, slant, boundary = np.zeros(24), np.zeros(16), np.zeros(1), np.zeros(16)
+    sg = ends.get("source_ground_m")
+    sgf = ends.get("source_ground_factor")
+    lib().qmo_ray_terms(
+        n, profile["horizontal_m"], _p(arr["t"], D), _p(arr["ground_m"], D), _p(arr["ground_factor"], D),
+        _p(forest, D), _p(canopy, D), ct.size, _p(ct, D), _p(ch, D), _p(cb, U8), _p(cr, U8), _p(cf, U64),
+        ends["source_height_m"], int(sg is not None), 0.0 if sg is None else sg, ends["receiver_altitude_m"],
+        int(sgf is not None), 0.0 if sgf is None else sgf, ends.get("platform_half_width_m", 0.0),
+        ends.get("own_footprint", 0), _p(fav, D), _p(al, D), _p(periods, D), _p(states, D), _p(slant, D),
+        _p(boundary, D))
+    return dict(periods=periods.reshape(3, 8), states=states.reshape(2, 8), slant_m=slant[0],
+                boundary_db=boundary.reshape(2, 8))
+
+
+def vertical_path(profile, crossings, source_height_m, platform_half_width_m=0.0, own_footprint=0, capacity=4096):
+    arr = {k: np.ascontiguousarray(profile[k], np.float64) for k in ("t", "ground_m", "ground_factor")}
+    ct = np.array([c[0] for c in crossings], np.float64)
+    ch = np.array([c[1] for c in crossings], np.float64)
+    cb = np.array([bool(c[2]) for c in crossings], np.uint8)
+    cr = np.array([bool(c[3]) for c in crossings], np.uint8)
+    cf = np.array([c[4] for c in crossings], np.uint64)
+    d, z, g = np.zeros(capacity), np.zeros(capacity), np.zeros(capacity)
+    tops, crests = np.zeros(2 * capacity), np.zeros(2 * capacity)
+    n, nt, nc = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
+    rc = lib().qmo_vertical_path(arr["t"].size, profile["horizontal_m"], _p(arr["t"], D), _p(arr["ground_m"], D),
+                                 _p(arr["ground_factor"], D), ct.size, _p(ct, D), _p(ch, D), _p(cb, U8), _p(cr, U8),
+                                 _p(cf, U64), source_height_m, platform_half_width_m, own_footprint, capacity,
+                                 _p(d, D), _p(z, D), _p(g, D), C.byref(n), _p(tops, D), C.byref(nt), _p(crests, D),
+                                 C.byref(nc))
+    assert rc == 0
+    k = n.value
+    return dict(d=d[:k], z=z[:k], g=g[:k], tops=tops[: 2 * nt.value].reshape(-1, 2),
+                crests=crests[: 2 * nc.value].reshape(-1, 2))
+
+
+def ground_at(terrain: abi.Terrain, x, y):
+    out = np.zeros(4)
+    failed = lib().qmo_ground_at(C.byref(terrain.c), x, y, _p(out, D))
+    return failed, out
+
+
+def fill_profile(terrain: abi.Terrain, source, receiver, capacity=1 << 16):
+    bufs = [np.zeros(capacity) for _ in range(5)]
+    n, horizontal = C.c_size_t(0), np.zeros(1)
+    failed = lib().qmo_fill_profile(C.byref(terrain.c), source[0], source[1], receiver[0], receiver[1], capacity,
+                                    *[_p(b, D) for b in bufs], C.byref(n), _p(horizontal, D))
+    k = n.value
+    return failed, dict(horizontal_m=horizontal[0], t=bufs[0][:k], ground_m=bufs[1][:k], ground_factor=bufs[2][:k],
+                        forest_cover=bufs[3][:k], canopy_m=bufs[4][:k])
+
+
+CROSSING_DTYPE = np.dtype([("t", "<f8"), ("height_m", "<f8"), ("building", "u1"), ("bounds_roof", "u1"),
+                           ("footprint_id", "<u8"), ("outline", "<u4"), ("edge", "<u4")])
+
+
+def crossings(obstacles: abi.Obstacles, start, end, brute_force=False, capacity=4096):
+    t, h = np.zeros(capacity), np.zeros(capacity)
+    b, r = np.zeros(capacity, np.uint8), np.zeros(capacity, np.uint8)
+    f = np.zeros(capacity, np.uint64)
+    o, e = np.zeros(capacity, np.uint32), np.zeros(capacity, np.uint32)
+    n = C.c_size_t(0)
+    failed = lib().qmo_crossings(C.byref(obstacles.c), start[0], start[1], end[0], end[1], capacity, _p(t, D),
+                                 _p(h, D), _p(b, U8), _p(r, U8), _p(f, U64), _p(o, U32), _p(e, U32), C.byref(n),
+                                 int(brute_force))
+    k = n.value
+    out = np.zeros(k, CROSSING_DTYPE)
+    if failed == 0:
+        out["t"], out["height_m"], out["building"], out["bounds_roof"] = t[:k], h[:k], b[:k], r[:k]
+        out["footprint_id"], out["outline"], out["edge"] = f[:k], o[:k], e[:k]
+    return failed, out
+
+
+def receivers(scene: abi.Scene, points):
+    points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+    n = points.size
+    altitude, alpha, fav = np.zeros(n), np.zeros((n, 8)), np.zeros((n, 3, 16))
+    failed = np.zeros(n, np.uint32)
+    lib().qmo_receivers(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), C.byref(scene.weather.c),
+                        points.ctypes.data, n, _p(altitude, D), _p(alpha, D), _p(fav, D), _p(failed, U32))
+    return dict(altitude_m=altitude, alpha=alpha, favourable=fav, failed=failed)
+
+
+def evaluate_pairs(scene: abi.Scene, points, sources, pair_point, pair_source, pair_weight=None, threads=None,
+                   point_bands=True):
+    points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+    sources = np.ascontiguousarray(sources, abi.SOURCE_DTYPE)
+    pp = np.ascontiguousarray(pair_point, np.uint32)
+    ps = np.ascontiguousarray(pair_source, np.uint32)
+    pw = None if pair_weight is None else np.ascontiguousarray(pair_weight, np.float32)
+    n = pp.size
+    sums = np.zeros((n, abi.PAIR_SUMS))
+    failed = np.zeros(n, np.uint32)
+    bands = np.zeros((points.size, abi.LAYERS, 3, 8)) if point_bands else None
+    work = np.zeros(6, np.uint64)
+    threads = threads or os.cpu_count() or 1
+    rc = lib().qmo_evaluate_pairs(
+        C.byref(scene.terrain.c), C.byref(scene.obstacles.c), C.byref(scene.weather.c), points.ctypes.data,
+        points.size, sources.ctypes.data, sources.size, _p(pp, U32), _p(ps, U32),
+        None if pw is None else _p(pw, F32), n, _p(sums, D), _p(failed, U32),
+        None if bands is None else _p(bands, D), threads, _p(work, U64))
+    if rc != 0:
+        raise ValueError("oracle: pair index out of range")
+    keys = ("vertices", "cells", "cells_occupied", "edges", "crossings", "candidates")
+    return dict(sums=sums, failed=failed, point_bands=bands, work=dict(zip(keys, (int(w) for w in work))))
+
+
+def paint_points(scene: abi.Scene, points, sources, cutoff_m, threads=None):
+    points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+    sources = np.ascontiguousarray(sources, abi.SOURCE_DTYPE)
+    n = points.size
+    energy = np.zeros((n, 3))
+    paths = np.zeros(n, np.uint64)
+    failures = np.zeros(n, np.uint32)
+    threads = threads or os.cpu_count() or 1
+    lib().qmo_paint_points(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), C.byref(scene.weather.c),
+                           points.ctypes.data, n, sources.ctypes.data, sources.size, cutoff_m, _p(energy, D),
+                           _p(paths, U64), _p(failures, U32), threads)
+    return dict(period_energy=energy, paths=paths, failures=failures)

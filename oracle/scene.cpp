@@ -1,0 +1,493 @@
+// TEST INFRASTRUCTURE — FP64 oracle: the ground under a ray, its obstacle crossings, the weather of
+// a receiver and one receiver-source pair end to end, restated from the reference's Rust
+// (popup/src/scene.rs, popup/src/obstacles/crossings.rs, physics/src/weather.rs,
+// popup/src/evaluate.rs, paint/src/exact.rs) on the planar scene of qm_scene.h. Not part of the
+// product path.
+#include "scene.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <thread>
+
+namespace qmo {
+
+// tiles/src/terrain.rs:36-38
+static double height_m_of_code(uint32_t code) { return -500.0 + (double)code / 5.0; }
+
+struct Node {
+    uint32_t height_code, impervious, forest, canopy;
+};
+
+static bool node_at(const qm_terrain& T, int64_t row, int64_t column, Node& out) {
+    if (row < 0 || column < 0 || row >= (int64_t)T.rows || column >= (int64_t)T.columns) return false;
+    const size_t index = (size_t)row * T.columns + (size_t)column;
+    const uint32_t v = T.nodes[index];
+    out = Node{v & 0xffffu, (v >> 16) & 0xffu, (v >> 24) & 0xffu, T.canopy_m ? (uint32_t)T.canopy_m[index] : 0u};
+    return true;
+}
+
+static bool node_valid(const Node& n) { return n.height_code != 0xffffu && n.impervious <= 100u; }
+
+// tiles/src/terrain.rs:342-385 Terrain::sample_at, the fractions measured north and east from
+// the cell's south-west node.
+int ground_at(const qm_terrain& T, double x, double y, GroundSample& out) {
+    const double cp = (x - T.origin_x_m) / T.spacing_m;
+    const double rp = (y - T.origin_y_m) / T.spacing_m;
+    const double column = std::floor(cp), row = std::floor(rp);
+    const double cf = cp - column, rf = rp - row;
+    const int64_t c = (int64_t)column, r = (int64_t)row;
+    const int64_t nc = c + (cf > 0.0 ? 1 : 0), nr = r + (rf > 0.0 ? 1 : 0);
+    Node corners[4];
+    if (!node_at(T, r, c, corners[0]) || !node_at(T, r, nc, corners[1]) || !node_at(T, nr, c, corners[2]) ||
+        !node_at(T, nr, nc, corners[3]))
+        return QM_FAILED_NO_TERRAIN;
+    const Node& nearest = corners[(rf >= 0.5 ? 2 : 0) + (cf >= 0.5 ? 1 : 0)];
+    if (nearest.forest > 100u) return QM_FAILED_NO_TERRAIN;
+    for (const Node& n : corners)
+        if (!node_valid(n)) return QM_FAILED_NO_TERRAIN;
+    const double w[4] = {(1.0 - rf) * (1.0 - cf), (1.0 - rf) * cf, rf * (1.0 - cf), rf * cf};
+    double height = 0.0, impervious = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        height += w[k] * height_m_of_code(corners[k].height_code);
+        impervious += w[k] * (double)corners[k].impervious;
+    }
+    out.height_m = height;
+    out.ground_factor = 1.0 - impervious / 100.0;
+    out.forest_cover = (double)nearest.forest / 100.0;
+    out.canopy_m = (double)nearest.canopy;
+    return 0;
+}
+
+// popup/src/scene.rs:532-561 edge_sample
+static int edge_sample(bool has_a, const Node& a_in, bool has_b, const Node& b_in, double f, GroundSample& out) {
+    Node a, b;
+    if (has_a && has_b) {
+        a = a_in;
+        b = b_in;
+    } else if (has_a && f <= 0.0) {
+        a = b = a_in;
+    } else if (has_b && f >= 1.0) {
+        a = b = b_in;
+    } else {
+        return QM_FAILED_NO_TERRAIN;
+    }
+    const Node& nearer = f < 0.5 ? a : b;
+    if (nearer.forest > 100u || !node_valid(a) || !node_valid(b)) return QM_FAILED_NO_TERRAIN;
+    const double ha = height_m_of_code(a.height_code), hb = height_m_of_code(b.height_code);
+    out.height_m = ha + f * (hb - ha);
+    const double ia = (double)a.impervious, ib = (double)b.impervious;
+    out.ground_factor = 1.0 - (ia + f * (ib - ia)) / 100.0;
+    out.forest_cover = (double)nearer.forest / 100.0;
+    out.canopy_m = (double)nearer.canopy;
+    return 0;
+}
+
+static void push_sample(Profile& profile, double t, const GroundSample& s) {
+    profile.push(t, s.height_m, s.ground_factor, s.forest_cover, s.canopy_m);
+}
+
+// popup/src/scene.rs:333-420 Ground::walk on the planar lattice: a vertex wherever the ray
+// a -> b crosses a row or column line strictly between fractions `from` and `to`, read between
+// that line's two nodes; at a node (both lines at once) the row line's reading, both advanced.
+static int walk(const qm_terrain& T, const double a[2], const double b[2], double from, double to, Profile& profile) {
+    const double ca = (a[0] - T.origin_x_m) / T.spacing_m, cb = (b[0] - T.origin_x_m) / T.spacing_m;
+    const double ra = (a[1] - T.origin_y_m) / T.spacing_m, rb = (b[1] - T.origin_y_m) / T.spacing_m;
+    const double columns = cb - ca, rise = rb - ra;
+    const double column_from = ca + from * columns, row_from = ra + from * rise;
+    double next_column, column_step;
+    if (columns > 0.0) {
+        next_column = std::floor(column_from) + 1.0;
+        column_step = 1.0;
+    } else {
+        next_column = std::ceil(column_from) - 1.0;
+        column_step = -1.0;
+    }
+    // `south`: the south row of the cell the ray is in; a start on a row line belongs to the
+    // cell the ray heads into.
+    double next_row, south;
+    if (rise > 0.0) {
+        next_row = std::floor(row_from) + 1.0;
+        south = std::floor(row_from);
+    } else if (rise < 0.0) {
+        next_row = std::ceil(row_from) - 1.0;
+        south = std::ceil(row_from) - 1.0;
+    } else {
+        next_row = 0.0;
+        south = std::floor(row_from);
+    }
+    const double inf = std::numeric_limits<double>::infinity();
+    for (;;) {
+        const double t_column = columns != 0.0 ? (next_column - ca) / columns : inf;
+        const double t_row = rise != 0.0 ? (next_row - ra) / rise : inf;
+        const double t = std::min(t_column, t_row);
+        if (t >= to) break;
+        Node n0{}, n1{};
+        bool h0, h1;
+        double f;
+        if (t_row <= t_column) {
+            const double c = ca + t * columns;
+            const double west = std::floor(c);
+            f = c - west;
+            h0 = node_at(T, (int64_t)next_row, (int64_t)west, n0);
+            h1 = node_at(T, (int64_t)next_row, (int64_t)west + 1, n1);
+        } else {
+            const double r = ra + t * rise;
+            f = std::min(std::max(r - south, 0.0), 1.0);
+            h0 = node_at(T, (int64_t)south, (int64_t)next_column, n0);
+            h1 = node_at(T, (int64_t)south + 1, (int64_t)next_column, n1);
+        }
+        GroundSample sample;
+        const int failed = edge_sample(h0, n0, h1, n1, f, sample);
+        if (failed) return failed;
+        push_sample(profile, t, sample);
+        if (t_row <= t_column) {
+            if (rise > 0.0) {
+                south = next_row;
+                next_row += 1.0;
+            } else {
+                south = next_row - 1.0;
+                next_row -= 1.0;
+            }
+        }
+        if (t_column <= t_row) next_column += column_step;
+    }
+    return 0;
+}
+
+// popup/src/scene.rs:242-309 Ground::fill_profile (one lattice: the synthetic scene has no fine
+// terrain, the reference's `fine_root` None branch).
+int fill_profile(const qm_terrain& T, const double source[2], const double receiver[2], Profile& profile) {
+    const double offset[2] = {source[0] - receiver[0], source[1] - receiver[1]};
+    const double length = std::hypot(offset[0], offset[1]);
+    profile.clear(std::max(length, 1.0));
+    GroundSample start, end;
+    int failed = ground_at(T, source[0], source[1], start);
+    if (failed) return failed;
+    push_sample(profile, 0.0, start);
+    failed = walk(T, source, receiver, 0.0, 1.0, profile);
+    if (failed) return failed;
+    failed = ground_at(T, receiver[0], receiver[1], end);
+    if (failed) return failed;
+    push_sample(profile, 1.0, end);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// popup/src/obstacles/mod.rs:61-76 Lattice::steps (snapped to a lattice point within 1e-6)
+static double snap(double steps) {
+    const double nearest = std::round(steps);
+    return std::fabs(steps - nearest) < 1e-6 ? nearest : steps;
+}
+void steps_of(const qm_obstacles& O, double x, double y, double out[2]) {
+    out[0] = snap(O.origin_steps_x + x / O.metres_per_step);
+    out[1] = snap(O.origin_steps_y - y / O.metres_per_step);
+}
+
+// crossings.rs:13-28
+static bool segment_intersection_t(const double start[2], const double delta[2], const double a[2], const double b[2],
+                                   double& t_out) {
+    const double ex = b[0] - a[0], ey = b[1] - a[1];
+    const double denominator = delta[0] * ey - delta[1] * ex;
+    if (denominator == 0.0) return false;
+    const double wx = a[0] - start[0], wy = a[1] - start[1];
+    const double t = (wx * ey - wy * ex) / denominator;
+    const double u = (wx * delta[1] - wy * delta[0]) / denominator;
+    if (t > 0.0 && t < 1.0 && u >= 0.0 && u <= 1.0) {
+        t_out = t;
+        return true;
+    }
+    return false;
+}
+
+static void vertex_steps(const qm_obstacles& O, size_t index, double out[2]) {
+    out[0] = (double)O.vertices[2 * index];
+    out[1] = (double)O.vertices[2 * index + 1];
+}
+
+// crossings.rs:80-111 SceneTile::edge_bounds_roof
+static bool edge_bounds_roof(const qm_obstacles& O, const double start[2], const double delta[2], size_t edge,
+                             size_t first, size_t count) {
+    auto side = [&](size_t index) {
+        double v[2];
+        vertex_steps(O, index, v);
+        return delta[0] * (v[1] - start[1]) - delta[1] * (v[0] - start[0]);
+    };
+    const double a = side(edge), b = side(edge + 1);
+    if (a != 0.0 && b != 0.0) return true;
+    const size_t distinct = count - 1;
+    const size_t vertex = ((a == 0.0 ? edge : edge + 1) - first) % distinct;
+    double n0 = 0.0, n1 = 0.0;
+    for (size_t offset = 1; offset < distinct; ++offset) {
+        if (n0 == 0.0) n0 = side(first + (vertex + distinct - offset) % distinct);
+        if (n1 == 0.0) n1 = side(first + (vertex + offset) % distinct);
+        if (n0 != 0.0 && n1 != 0.0) break;
+    }
+    return n0 * n1 < 0.0;
+}
+
+// crossings.rs:34-56 sort_and_deduplicate; ties of (t, footprint) ordered by outline and edge so
+// that the kept crossing is one fixed choice.
+void sort_and_deduplicate(std::vector<RawCrossing>& list) {
+    std::sort(list.begin(), list.end(), [](const RawCrossing& a, const RawCrossing& b) {
+        if (a.c.t != b.c.t) return a.c.t < b.c.t;
+        if (a.c.footprint_id != b.c.footprint_id) return a.c.footprint_id < b.c.footprint_id;
+        if (a.outline != b.outline) return a.outline < b.outline;
+        return a.edge < b.edge;
+    });
+    size_t kept = 0;
+    for (size_t i = 0; i < list.size(); ++i) {
+        const RawCrossing candidate = list[i];
+        bool merged = false;
+        for (size_t k = kept; k-- > 0;) {
+            if (!(candidate.c.t - list[k].c.t < 1e-9)) break;
+            if (list[k].c.footprint_id == candidate.c.footprint_id) {
+                list[k].c.bounds_roof = list[k].c.bounds_roof || candidate.c.bounds_roof;
+                merged = true;
+                break;
+            }
+        }
+        if (!merged) list[kept++] = candidate;
+    }
+    list.resize(kept);
+}
+
+static RawCrossing make_crossing(const qm_obstacles& O, const double start[2], const double delta[2], uint32_t outline,
+                                 size_t edge_vertex, double t) {
+    const qm_outline& record = O.outlines[outline];
+    RawCrossing r;
+    r.c.t = t;
+    r.c.height_m = (double)record.height_dm / 10.0;
+    r.c.building = record.kind == QM_KIND_BUILDING;
+    r.c.bounds_roof = r.c.building &&
+                      edge_bounds_roof(O, start, delta, edge_vertex, record.first_vertex, record.vertex_count);
+    r.c.footprint_id = record.footprint_id;
+    r.outline = outline;
+    r.edge = (uint32_t)edge_vertex;
+    return r;
+}
+
+// crossings.rs:163-230 Scene::crossings: an Amanatides-Woo walk over the cells, the padded piece
+// box filter (crossings.rs:60-75) and the exact test on each listed edge.
+int crossings_raw(const qm_obstacles& O, const double from[2], const double to[2], std::vector<RawCrossing>& out,
+                  WorkCount* work) {
+    out.clear();
+    double start[2], end[2];
+    steps_of(O, from[0], from[1], start);
+    steps_of(O, to[0], to[1], end);
+    const double delta[2] = {end[0] - start[0], end[1] - start[1]};
+    const double cs = (double)QM_CELL_STEPS;
+    int64_t cell[2] = {(int64_t)std::floor(start[0] / cs), (int64_t)std::floor(start[1] / cs)};
+    const int64_t last[2] = {(int64_t)std::floor(end[0] / cs), (int64_t)std::floor(end[1] / cs)};
+    const int64_t step[2] = {delta[0] >= 0.0 ? 1 : -1, delta[1] >= 0.0 ? 1 : -1};
+    const double inf = std::numeric_limits<double>::infinity();
+    double t_delta[2], t_max[2];
+    for (int axis = 0; axis < 2; ++axis) {
+        t_delta[axis] = delta[axis] != 0.0 ? std::fabs(cs / delta[axis]) : inf;
+        if (delta[axis] == 0.0) {
+            t_max[axis] = inf;
+        } else {
+            const double boundary = (double)(cell[axis] + (delta[axis] >= 0.0 ? 1 : 0)) * cs;
+            t_max[axis] = std::fabs((boundary - start[axis]) / delta[axis]);
+        }
+    }
+    const double pad = 1e-9 * (1.0 + std::fabs(delta[0]) + std::fabs(delta[1]));
+    double t_enter = 0.0;
+    for (;;) {
+        const int64_t gx = cell[0] - O.cell_x0, gy = cell[1] - O.cell_y0;
+        if (gx < 0 || gy < 0 || gx >= (int64_t)O.cells_x || gy >= (int64_t)O.cells_y) return QM_FAILED_NOT_READ;
+        const size_t index = (size_t)gy * O.cells_x + (size_t)gx;
+        const uint32_t r0 = O.cell_first_run[index], r1 = O.cell_first_run[index + 1];
+        if (work) work->cells += 1;
+        if (r1 > r0) {
+            if (work) work->cells_occupied += 1;
+            const double t_exit = std::min(std::min(t_max[0], t_max[1]), 1.0);
+            const double te = std::min(std::max(t_enter, 0.0), 1.0), tx = std::min(std::max(t_exit, 0.0), 1.0);
+            const double p[2] = {start[0] + delta[0] * te, start[1] + delta[1] * te};
+            const double q[2] = {start[0] + delta[0] * tx, start[1] + delta[1] * tx};
+            int32_t low[2], high[2];
+            for (int axis = 0; axis < 2; ++axis) {
+                low[axis] = (int32_t)std::floor(std::min(p[axis], q[axis]) - pad);
+                high[axis] = (int32_t)std::ceil(std::max(p[axis], q[axis]) + pad);
+            }
+            for (uint32_t ri = r0; ri < r1; ++ri) {
+                const qm_run& run = O.runs[ri];
+                const size_t first = (size_t)O.outlines[run.outline].first_vertex + run.first_edge;
+                int32_t ax = O.vertices[2 * first], ay = O.vertices[2 * first + 1];
+                for (size_t next = first + 1; next <= first + run.edge_count; ++next) {
+                    const int32_t bx = O.vertices[2 * next], by = O.vertices[2 * next + 1];
+                    if (work) work->edges += 1;
+                    const bool meets = high[0] >= std::min(ax, bx) && std::max(ax, bx) >= low[0] &&
+                                       high[1] >= std::min(ay, by) && std::max(ay, by) >= low[1];
+                    double t;
+                    const double av[2] = {(double)ax, (double)ay}, bv[2] = {(double)bx, (double)by};
+                    if (meets && segment_intersection_t(start, delta, av, bv, t))
+                        out.push_back(make_crossing(O, start, delta, run.outline, next - 1, t));
+                    ax = bx;
+                    ay = by;
+                }
+            }
+        }
+        if (cell[0] == last[0] && cell[1] == last[1]) break;
+        t_enter = std::min(t_max[0], t_max[1]);
+        int axis;
+        if (cell[0] == last[0]) axis = 1;
+        else if (cell[1] == last[1] || t_max[0] < t_max[1]) axis = 0;
+        else axis = 1;
+        t_max[axis] += t_delta[axis];
+        cell[axis] += step[axis];
+    }
+    sort_and_deduplicate(out);
+    if (work) work->crossings += out.size();
+    return 0;
+}
+
+// Brute force: every edge of every outline, no cells and no box filter; the crossing set a correct
+// cell walk must reproduce exactly (test support for the cell walk).
+void crossings_brute_force(const qm_obstacles& O, const double from[2], const double to[2],
+                           std::vector<RawCrossing>& out) {
+    out.clear();
+    double start[2], end[2];
+    steps_of(O, from[0], from[1], start);
+    steps_of(O, to[0], to[1], end);
+    const double delta[2] = {end[0] - start[0], end[1] - start[1]};
+    for (uint32_t o = 0; o < O.outlines_count; ++o) {
+        const qm_outline& record = O.outlines[o];
+        for (size_t v = record.first_vertex; v + 1 < (size_t)record.first_vertex + record.vertex_count; ++v) {
+            double a[2], b[2], t;
+            vertex_steps(O, v, a);
+            vertex_steps(O, v + 1, b);
+            if (segment_intersection_t(start, delta, a, b, t)) out.push_back(make_crossing(O, start, delta, o, v, t));
+        }
+    }
+    sort_and_deduplicate(out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// physics/src/weather.rs:175-214 corners + place_of on the planar weather lattice (clamped at its
+// edges where the globe's table wraps in longitude).
+void place_weather(const qm_weather& W, double x, double y, double favourable[PERIODS][SECTORS], double alpha[BANDS]) {
+    const double yy = std::min(std::max((y - W.origin_y_m) / W.spacing_m, 0.0), (double)(W.rows - 1));
+    const double xx = std::min(std::max((x - W.origin_x_m) / W.spacing_m, 0.0), (double)(W.columns - 1));
+    const uint32_t row = std::min((uint32_t)std::floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
+    const uint32_t column = std::min((uint32_t)std::floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
+    const double fy = yy - row, fx = xx - column;
+    const uint32_t r1 = W.rows >= 2 ? row + 1 : row, c1 = W.columns >= 2 ? column + 1 : column;
+    const uint8_t* n[4] = {W.nodes + ((size_t)row * W.columns + column) * QM_WEATHER_NODE_BYTES,
+                           W.nodes + ((size_t)row * W.columns + c1) * QM_WEATHER_NODE_BYTES,
+                           W.nodes + ((size_t)r1 * W.columns + column) * QM_WEATHER_NODE_BYTES,
+                           W.nodes + ((size_t)r1 * W.columns + c1) * QM_WEATHER_NODE_BYTES};
+    const double w[4] = {(1.0 - fy) * (1.0 - fx), (1.0 - fy) * fx, fy * (1.0 - fx), fy * fx};
+    for (int p = 0; p < PERIODS; ++p)
+        for (int s = 0; s < SECTORS; ++s) {
+            double total = 0.0;
+            for (int k = 0; k < 4; ++k) total += w[k] * (double)n[k][p * SECTORS + s];
+            favourable[p][s] = total / 100.0;
+        }
+    for (int b = 0; b < BANDS; ++b) {
+        double total = 0.0;
+        const int at = PERIODS * SECTORS + 2 * b;
+        for (int k = 0; k < 4; ++k) total += w[k] * (double)((uint32_t)n[k][at] | ((uint32_t)n[k][at + 1] << 8));
+        alpha[b] = total / 100.0;
+    }
+}
+
+// weather.rs:61-70 FavourableProbability::at
+double favourable_at(const double favourable[PERIODS][SECTORS], int period, double azimuth_rad) {
+    double bearing = std::fmod(90.0 - azimuth_rad * (180.0 / M_PI), 360.0);
+    if (bearing < 0.0) bearing += 360.0;
+    const double position = bearing / (360.0 / SECTORS);
+    const size_t lower = (size_t)std::floor(position) % SECTORS;
+    const size_t upper = (lower + 1) % SECTORS;
+    const double fraction = position - std::floor(position);
+    const double* row = favourable[period];
+    return row[lower] + fraction * (row[upper] - row[lower]);
+}
+
+// paint/src/exact.rs:95-118 Point::with_receiver_properties: ground + 4 m and the weather there.
+int receiver_of(const Scene& S, const qm_point& point, Receiver& out) {
+    GroundSample g;
+    const int failed = ground_at(S.terrain, point.x_m, point.y_m, g);
+    out.x = point.x_m;
+    out.y = point.y_m;
+    out.own_footprint = point.own_footprint;
+    out.altitude_m = failed ? 0.0 : g.height_m + QM_RECEIVER_HEIGHT_M;
+    place_weather(S.weather, point.x_m, point.y_m, out.favourable, out.alpha);
+    return failed;
+}
+
+// popup/src/evaluate.rs:171-221 source_rays (a point: one ray), 300-330 ray, 119-152 add_ray;
+// paint/src/exact.rs:186-205 whole_energy.
+int evaluate_pair(const Scene& S, const Receiver& receiver, const qm_source& source, PairScratch& s, PairResult& out,
+                  WorkCount* work) {
+    for (int p = 0; p < PERIODS; ++p) {
+        out.mixed[p] = 0.0;
+        out.states[0][p] = out.states[1][p] = 0.0;
+        for (int b = 0; b < BANDS; ++b) out.bands[p][b] = 0.0;
+    }
+    const double src[2] = {source.x_m, source.y_m};
+    const double rcv[2] = {receiver.x, receiver.y};
+    int failed = fill_profile(S.terrain, src, rcv, s.profile);
+    if (failed) return failed;
+    if (work) work->vertices += s.profile.t.size();
+    failed = crossings_raw(S.obstacles, src, rcv, s.raw, work);
+    if (failed) return failed;
+    s.crossings.clear();
+    for (const RawCrossing& r : s.raw)  // evaluate.rs:293-297 without_own_walls
+        if (receiver.own_footprint == 0 || r.c.footprint_id != receiver.own_footprint) s.crossings.push_back(r.c);
+    const double azimuth = std::atan2(receiver.y - source.y_m, receiver.x - source.x_m);
+    double p[PERIODS];
+    for (int period = 0; period < PERIODS; ++period) p[period] = favourable_at(receiver.favourable, period, azimuth);
+    RayEnds ends;
+    ends.source_height_m = source.height_m;
+    ends.has_source_ground = false;
+    ends.source_ground_m = 0.0;
+    ends.receiver_altitude_m = receiver.altitude_m;
+    ends.has_source_ground_factor = source.ground_percent != QM_GROUND_FROM_TERRAIN;
+    ends.source_ground_factor = (double)source.ground_percent / 100.0;
+    ends.platform_half_width_m = source.platform_half_width_m;
+    ends.own_footprint = source.footprint_id;
+    ray_terms(s.profile, s.crossings, ends, p, receiver.alpha, s.ray, s.terms, work);
+    // evaluate.rs:195-203: the point divergence at the candidate's distance.
+    const double horizontal = std::hypot(source.x_m - receiver.x, source.y_m - receiver.y);
+    const double ground0 = s.profile.ground_m[0];
+    const double d = std::max(std::hypot(std::max(horizontal, (double)source.exclusion_radius_m),
+                                         receiver.altitude_m - (ground0 + (double)source.height_m)),
+                              1.0);
+    const double weight = 1.0 / (d * d * energy(11.0));
+    const Transfer& tr = s.terms.transfer;
+    for (int period = 0; period < PERIODS; ++period)
+        for (int b = 0; b < BANDS; ++b) {
+            const double power = weight * (double)source.energy[period][b];
+            out.bands[period][b] += power * tr.periods[period][b];
+            for (int st = 0; st < 2; ++st) out.states[st][period] += power * tr.states[st][b];
+        }
+    for (int period = 0; period < PERIODS; ++period) {
+        double sum = 0.0;
+        for (int b = 0; b < BANDS; ++b) sum += out.bands[period][b];
+        out.mixed[period] = sum;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+void parallel_for(size_t n, int threads, const std::function<void(size_t, int)>& body) {
+    if (threads <= 1 || n < 2) {
+        for (size_t i = 0; i < n; ++i) body(i, 0);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int k = 0; k < threads; ++k)
+        pool.emplace_back([&, k] {
+            for (;;) {
+                const size_t i = next.fetch_add(1);
+                if (i >= n) break;
+                body(i, k);
+            }
+        });
+    for (std::thread& t : pool) t.join();
+}
+
+}  // namespace qmo

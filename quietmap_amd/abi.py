@@ -1,0 +1,193 @@
+"""ctypes mirrors of the scene ABI in ``csrc/qm_scene.h`` and numpy record dtypes of its arrays.
+
+The scene is handed over as plain pointers and sizes, as ``paint-gpu/src/square_upload.rs:18-287``
+hands the reference's card its flat buffers; these classes hold numpy arrays alive and expose the
+C structs that point into them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+BANDS = 8
+PERIODS = 3
+SECTORS = 16
+LAYERS = 6
+PAIR_SUMS = 9
+CELL_STEPS = 256
+RECEIVER_HEIGHT_M = 4.0
+GROUND_FROM_TERRAIN = 255
+WEATHER_NODE_BYTES = 64
+KIND_BUILDING = 0
+KIND_BARRIER = 1
+
+FAILED_NOT_READ = 1
+FAILED_NO_TERRAIN = 2
+FAILED_CAPACITY = 4
+
+RUN_DTYPE = np.dtype([("outline", "<u4"), ("first_edge", "<u2"), ("edge_count", "<u2")])
+OUTLINE_DTYPE = np.dtype(
+    [
+        ("footprint_id", "<u8"),
+        ("first_vertex", "<u4"),
+        ("vertex_count", "<u4"),
+        ("height_dm", "<u2"),
+        ("kind", "u1"),
+        ("pad", "u1", (5,)),
+    ]
+)
+SOURCE_DTYPE = np.dtype(
+    [
+        ("x_m", "<f8"),
+        ("y_m", "<f8"),
+        ("height_m", "<f4"),
+        ("platform_half_width_m", "<f4"),
+        ("exclusion_radius_m", "<f4"),
+        ("ground_percent", "u1"),
+        ("layer", "u1"),
+        ("pad", "u1", (2,)),
+        ("footprint_id", "<u8"),
+        ("energy", "<f4", (PERIODS, BANDS)),
+    ]
+)
+POINT_DTYPE = np.dtype([("x_m", "<f8"), ("y_m", "<f8"), ("own_footprint", "<u8")])
+assert RUN_DTYPE.itemsize == 8 and OUTLINE_DTYPE.itemsize == 24
+assert SOURCE_DTYPE.itemsize == 136 and POINT_DTYPE.itemsize == 24
+
+
+def _ptr(array, ctype):
+    if array is None:
+        return C.cast(None, C.POINTER(ctype))
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+class CTerrain(C.Structure):
+    _fields_ = [
+        ("origin_x_m", C.c_double),
+        ("origin_y_m", C.c_double),
+        ("spacing_m", C.c_double),
+        ("columns", C.c_uint32),
+        ("rows", C.c_uint32),
+        ("nodes", C.c_void_p),
+        ("canopy_m", C.c_void_p),
+    ]
+
+
+class CObstacles(C.Structure):
+    _fields_ = [
+        ("origin_steps_x", C.c_double),
+        ("origin_steps_y", C.c_double),
+        ("metres_per_step", C.c_double),
+        ("cell_x0", C.c_int32),
+        ("cell_y0", C.c_int32),
+        ("cells_x", C.c_uint32),
+        ("cells_y", C.c_uint32),
+        ("runs_count", C.c_uint32),
+        ("outlines_count", C.c_uint32),
+        ("vertices_count", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("cell_first_run", C.c_void_p),
+        ("runs", C.c_void_p),
+        ("outlines", C.c_void_p),
+        ("vertices", C.c_void_p),
+    ]
+
+
+class CWeather(C.Structure):
+    _fields_ = [
+        ("origin_x_m", C.c_double),
+        ("origin_y_m", C.c_double),
+        ("spacing_m", C.c_double),
+        ("columns", C.c_uint32),
+        ("rows", C.c_uint32),
+        ("nodes", C.c_void_p),
+    ]
+
+
+class Terrain:
+    """A terrain lattice: ``nodes`` (rows, columns) u32 = height code | imperviousness << 16 |
+    forest << 24, ``canopy_m`` (rows, columns) u8; row r at y = origin_y + r*spacing."""
+
+    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, canopy_m=None):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        rows, columns = self.nodes.shape
+        self.canopy_m = (
+            np.zeros((rows, columns), np.uint8) if canopy_m is None else np.ascontiguousarray(canopy_m, np.uint8)
+        )
+        self.origin_x_m, self.origin_y_m, self.spacing_m = float(origin_x_m), float(origin_y_m), float(spacing_m)
+        self.c = CTerrain(
+            self.origin_x_m,
+            self.origin_y_m,
+            self.spacing_m,
+            columns,
+            rows,
+            self.nodes.ctypes.data,
+            self.canopy_m.ctypes.data,
+        )
+
+    @staticmethod
+    def pack(height_m, impervious_percent, forest_percent=None):
+        """Nodes from heights (m, coded at 0.2 m as tiles/src/terrain.rs:41-45) and percentages."""
+        code = np.clip(np.round((np.asarray(height_m, np.float64) + 500.0) * 5.0), 0, 65534).astype(np.uint32)
+        imp = np.asarray(impervious_percent).astype(np.uint32)
+        forest = np.zeros_like(imp) if forest_percent is None else np.asarray(forest_percent).astype(np.uint32)
+        return code | (imp << 16) | (forest << 24)
+
+
+class Obstacles:
+    """The obstacle cell index (see qm_scene.h)."""
+
+    def __init__(self, origin_steps, metres_per_step, cell_origin, cells, cell_first_run, runs, outlines, vertices):
+        self.cell_first_run = np.ascontiguousarray(cell_first_run, np.uint32)
+        self.runs = np.ascontiguousarray(runs, RUN_DTYPE)
+        self.outlines = np.ascontiguousarray(outlines, OUTLINE_DTYPE)
+        self.vertices = np.ascontiguousarray(vertices, np.int32).reshape(-1, 2)
+        self.origin_steps = (float(origin_steps[0]), float(origin_steps[1]))
+        self.metres_per_step = float(metres_per_step)
+        self.cell_origin = (int(cell_origin[0]), int(cell_origin[1]))
+        self.cells = (int(cells[0]), int(cells[1]))
+        assert self.cell_first_run.size == self.cells[0] * self.cells[1] + 1
+        self.c = CObstacles(
+            self.origin_steps[0],
+            self.origin_steps[1],
+            self.metres_per_step,
+            self.cell_origin[0],
+            self.cell_origin[1],
+            self.cells[0],
+            self.cells[1],
+            self.runs.size,
+            self.outlines.size,
+            self.vertices.shape[0],
+            0,
+            self.cell_first_run.ctypes.data,
+            self.runs.ctypes.data if self.runs.size else None,
+            self.outlines.ctypes.data if self.outlines.size else None,
+            self.vertices.ctypes.data if self.vertices.size else None,
+        )
+
+
+class Weather:
+    """A weather lattice of 64-byte nodes (physics/src/weather.rs:19-27)."""
+
+    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes):
+        self.nodes = np.ascontiguousarray(nodes, np.uint8)
+        rows, columns, width = self.nodes.shape
+        assert width == WEATHER_NODE_BYTES
+        self.c = CWeather(float(origin_x_m), float(origin_y_m), float(spacing_m), columns, rows, self.nodes.ctypes.data)
+
+    @staticmethod
+    def node(percent, alpha_db_per_km):
+        """One node's bytes: percent [3][16] (0-100) and alpha per band (dB/km, coded in 0.01)."""
+        out = np.zeros(WEATHER_NODE_BYTES, np.uint8)
+        out[: PERIODS * SECTORS] = np.asarray(percent, np.uint8).reshape(-1)
+        codes = np.round(np.asarray(alpha_db_per_km, np.float64) * 100.0).astype("<u2")
+        out[PERIODS * SECTORS :] = codes.view(np.uint8)
+        return out
+
+
+class Scene:
+    """Terrain, obstacles and weather of one painted area."""
+
+    def __init__(self, terrain: Terrain, obstacles: Obstacles, weather: Weather):
+        self.terrain, self.obstacles, self.weather = terrain, obstacles, weather

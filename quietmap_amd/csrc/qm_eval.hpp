@@ -1,0 +1,442 @@
+// One receiver-source pair end to end on the device: popup/src/evaluate.rs:171-221 (a point's
+// one ray), 300-330 (the ray), 119-152 (add_ray) and paint/src/exact.rs:186-205 (whole_energy),
+// over physics/src/ray.rs:125-354. See qm_path.hpp for the pieces and the design.
+#pragma once
+
+#include "qm_path.hpp"
+
+namespace qm {
+
+// The per-path buffers (private to the work-item).
+struct PathScratch {
+    Crossing cross[CROSS_CAP];
+    Roof roofs[ROOF_CAP];
+    Profile profile;
+    StateStream state[2];
+};
+
+// The crest thinning of physics/src/ray.rs:360-385: the open group's best sample.
+struct CrestGroup {
+    bool open;
+    int group;
+    Cand best;
+};
+
+struct Emitter {
+    int next_top;  // next crossing whose top is not yet a candidate
+    int tops_resolved;
+};
+
+// Favourable probability of `period` for sound travelling along `azimuth` (weather.rs:61-70).
+QM_DEV float favourable_at(const float* row, float azimuth) {
+    float bearing = fmodf(90.0f - azimuth * (180.0f / 3.14159265358979f), 360.0f);
+    if (bearing < 0.0f) bearing += 360.0f;
+    const float position = bearing * (SECTORS / 360.0f);
+    const float fl = floorf(position);
+    const int lower = ((int)fl) % SECTORS;
+    const int upper = (lower + 1) % SECTORS;
+    const float fraction = position - fl;
+    return row[lower] + fraction * (row[upper] - row[lower]);
+}
+
+QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, const Cand& c, uint32_t& failed) {
+    if (!(c.x > 0.0f && c.x < length)) return;
+    state_candidate(w.state[0], S, R, c, failed);
+    state_candidate(w.state[1], S, R, c, failed);
+}
+
+// Emits every resolved top before x (the candidate merge of physics/src/cnossos/mod.rs:160-184:
+// a top first only when strictly nearer).
+QM_DEV void tops_before(PathScratch& w, Emitter& e, int n, P2f S, P2f R, float length, float x, uint32_t& failed) {
+    while (e.next_top < n) {
+        const Crossing& c = w.cross[e.next_top];
+        if (c.flags & 4u) {
+            ++e.next_top;
+            continue;
+        }
+        if (!(c.x < x) || e.next_top >= e.tops_resolved) break;
+        Cand k;
+        k.x = c.x;
+        k.z = c.top;
+        k.z_at = c.z_at;
+        k.g_at = c.g_at;
+        k.pz = c.pz;
+        k.pxz = c.pxz;
+        k.pg = c.pg;
+        candidate_out(w, S, R, length, k, failed);
+        ++e.next_top;
+    }
+}
+
+// Resolves (profile and prefix at x) every top before `x_limit` not yet resolved.
+QM_DEV void resolve_tops(PathScratch& w, Emitter& e, int n, float x_limit, bool all, uint32_t& failed) {
+    while (e.tops_resolved < n) {
+        Crossing& c = w.cross[e.tops_resolved];
+        if (!(c.flags & 4u)) {
+            if (!all && !(c.x < x_limit)) break;
+            Cand k;
+            if (!profile_query(w.profile, c.x, k)) failed |= QM_FAILED_CAPACITY;
+            c.z_at = k.z_at;
+            c.g_at = k.g_at;
+            c.pz = k.pz;
+            c.pxz = k.pxz;
+            c.pg = k.pg;
+        }
+        ++e.tops_resolved;
+    }
+}
+
+QM_DEV void crest_vertex(PathScratch& w, CrestGroup& cg, Emitter& e, int n, P2f S, P2f R, float length, float x,
+                         float z, uint32_t& failed) {
+    const bool middle = x > CREST_ENDS_M && x < length - CREST_ENDS_M;
+    const int group = (int)floorf(x / CREST_SPACING_M);
+    if (cg.open && (!middle || cg.group != group)) {
+        tops_before(w, e, n, S, R, length, cg.best.x, failed);
+        candidate_out(w, S, R, length, cg.best, failed);
+        cg.open = false;
+    }
+    if (!middle) {
+        Cand k;
+        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY;
+        k.x = x;
+        k.z = z;
+        tops_before(w, e, n, S, R, length, x, failed);
+        candidate_out(w, S, R, length, k, failed);
+        return;
+    }
+    if (!cg.open || z > cg.best.z) {
+        Cand k;
+        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY;
+        k.x = x;
+        k.z = z;
+        cg.best = k;
+        cg.group = group;
+        cg.open = true;
+    }
+}
+
+// A ray's foliage depth in one state (foliage.rs:36-64), streamed over the terrain samples.
+struct Foliage {
+    float depth[2];
+    float prev_x, prev_z[2], prev_w[2], prev_cover;
+};
+
+QM_DEV void foliage_sample(Foliage& f, const PathScratch& w, P2f S, P2f R, float length, float x, float ground,
+                           float cover, float canopy, bool first) {
+    float zs[2], ws[2];
+    for (int s = 0; s < 2; ++s) {
+        const float radius = w.state[s].radius;
+        zs[s] = S.z + (x / length) * (R.z - S.z) + sag_f(radius, S, R, x);
+        const float above = zs[s] - ground;
+        ws[s] = (above > 0.0f && above <= canopy) ? cover : 0.0f;
+    }
+    if (!first && (f.prev_cover > 0.0f || cover > 0.0f)) {
+        for (int s = 0; s < 2; ++s) {
+            const float dx = x - f.prev_x, dz = zs[s] - f.prev_z[s];
+            f.depth[s] += 0.5f * (f.prev_w[s] + ws[s]) * sqrtf(dx * dx + dz * dz);
+        }
+    }
+    f.prev_x = x;
+    f.prev_cover = cover;
+    for (int s = 0; s < 2; ++s) {
+        f.prev_z[s] = zs[s];
+        f.prev_w[s] = ws[s];
+    }
+}
+
+// terrain_at and ground_factor_at of physics/src/ray.rs:265-283 at x = t * length, from the two
+// lattice-line vertices bracketing it (platform-clamped as the terrain polyline is).
+QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, float length, float platform,
+                                float source_ground, const GroundSample& start, const GroundSample& end, float& z_out,
+                                float& g_out) {
+    float tp, tn;
+    GroundSample a, b;
+    bool ea, eb;
+    uint32_t failed = bracket_vertex(T, L, t, length, false, tp, a, ea);
+    failed |= bracket_vertex(T, L, t, length, true, tn, b, eb);
+    if (ea) a = start;
+    if (eb) b = end;
+    if (failed) return failed;
+    const float x = t * length;
+    const float xa = tp * length, xb = tn * length;
+    const float za = xa < platform ? fminf(a.z, source_ground) : a.z;
+    const float zb = xb < platform ? fminf(b.z, source_ground) : b.z;
+    const float f = xb > xa ? clampf((x - xa) / (xb - xa), 0.0f, 1.0f) : 0.0f;
+    z_out = za + f * (zb - za);
+    g_out = a.g + f * (b.g - a.g);
+    return 0;
+}
+
+QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver& rcv, const qm_source& src,
+                          PathScratch& w, PathResult& out) {
+    uint32_t failed = rcv.failed;
+    for (int p = 0; p < PERIODS; ++p) {
+        for (int b = 0; b < BANDS; ++b) out.bands[p][b] = 0.0f;
+        out.states[0][p] = out.states[1][p] = 0.0f;
+    }
+    out.failed = failed;
+    if (failed) return;
+    const qm_terrain& T = sc.terrain;
+    const double sx = src.x_m, sy = src.y_m, rx = rcv.x_m, ry = rcv.y_m;
+    if (!isfinite(sx) || !isfinite(sy) || !isfinite(rx) || !isfinite(ry)) {
+        out.failed = QM_FAILED_NOT_READ;
+        return;
+    }
+    const double length_d = hypot(sx - rx, sy - ry);
+    const float H = (float)fmax(length_d, 1.0);
+
+    // Crossings, the receiver's own walls dropped (evaluate.rs:293-297).
+    int n = 0;
+    failed |= find_crossings(sc.obstacles, sx, sy, rx, ry, w.cross, n);
+    if (failed & QM_FAILED_NOT_READ) {
+        out.failed = failed;
+        return;
+    }
+    for (int i = 0; i < n; ++i) {
+        w.cross[i].flags &= 3u;
+        if (rcv.own_footprint != 0 && w.cross[i].footprint == rcv.own_footprint) w.cross[i].flags |= 4u;
+    }
+    // Footprints containing a building's source never screen it (ray.rs:201-226).
+    if (src.footprint_id != 0) {
+        for (int i = 0; i < n; ++i) {
+            if (w.cross[i].flags & 4u) continue;
+            const uint64_t id = w.cross[i].footprint;
+            int count = 0;
+            for (int j = 0; j < n; ++j)
+                if (!(w.cross[j].flags & 4u) && (w.cross[j].flags & 2u) && w.cross[j].footprint == id) ++count;
+            if ((count & 1) || id == src.footprint_id) w.cross[i].flags |= 8u;
+        }
+        for (int i = 0; i < n; ++i)
+            if (w.cross[i].flags & 8u) w.cross[i].flags = (w.cross[i].flags & ~8u) | 4u;
+    }
+
+    // The ends (tiles/src/terrain.rs:342-385 at both ends of popup/src/scene.rs:242-309).
+    const Lattice L = lattice_of(T, sx, sy, rx, ry);
+    const double inv = 1.0 / T.spacing_m;
+    GroundSample s0, r1;
+    failed |= ground_at(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
+    failed |= ground_at(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+    if (failed & QM_FAILED_NO_TERRAIN) {
+        out.failed = failed;
+        return;
+    }
+    const float source_ground = s0.z;
+    const float source_alt = source_ground + src.height_m;
+    const float receiver_alt = rcv.altitude_m;
+    const float platform = src.platform_half_width_m;
+    const float source_g = src.ground_percent != QM_GROUND_FROM_TERRAIN ? (float)src.ground_percent * 0.01f : s0.g;
+
+    // Tops on the terrain polyline (ray.rs:304-316) and the roofs they bound (ray.rs:317-335).
+    int nroofs = 0;
+    for (int i = 0; i < n; ++i) {
+        Crossing& c = w.cross[i];
+        if (c.flags & 4u) continue;
+        const float t = (float)c.t;
+        c.x = (float)(c.t * (double)H);
+        float z, g;
+        failed |= terrain_bracket(T, L, t, H, platform, source_ground, s0, r1, z, g);
+        c.top = z + c.height_m;
+        c.z_at = z;  // terrain under the wall, until the walk resolves the profile there
+        c.g_at = g;
+    }
+    // Pair each footprint's bounding crossings in order of distance.
+    for (int i = 0; i < n; ++i) w.cross[i].flags &= ~16u;
+    for (int i = 0; i < n; ++i) {
+        Crossing& a = w.cross[i];
+        if ((a.flags & 4u) || !(a.flags & 2u) || (a.flags & 16u)) continue;
+        a.flags |= 16u;
+        int open = i;  // an entry waiting for its exit
+        for (int j = i + 1; j < n; ++j) {
+            Crossing& b = w.cross[j];
+            if ((b.flags & 4u) || !(b.flags & 2u) || (b.flags & 16u) || b.footprint != a.footprint) continue;
+            b.flags |= 16u;
+            if (open < 0) {
+                open = j;
+                continue;
+            }
+            if (nroofs >= ROOF_CAP) {
+                failed |= QM_FAILED_CAPACITY;
+            } else {
+                const Crossing& e = w.cross[open];
+                Roof r;
+                r.x0 = e.x;
+                r.x1 = b.x;
+                r.top0 = e.top;
+                r.top1 = b.top;
+                r.ground0 = e.z_at;
+                r.g0 = e.g_at;
+                r.ground1 = b.z_at;
+                r.g1 = b.g_at;
+                // Insert in closing order (x1, then x0), stable.
+                int k = nroofs;
+                while (k > 0 && (w.roofs[k - 1].x1 > r.x1 || (w.roofs[k - 1].x1 == r.x1 && w.roofs[k - 1].x0 > r.x0))) {
+                    w.roofs[k] = w.roofs[k - 1];
+                    --k;
+                }
+                w.roofs[k] = r;
+                ++nroofs;
+            }
+            open = -1;
+        }
+    }
+    // Clip in closing order (ray.rs:389-404).
+    {
+        float covered = -__builtin_inff();
+        float covered_ground = 0.0f, covered_g = 0.0f;
+        int kept = 0;
+        for (int i = 0; i < nroofs; ++i) {
+            Roof r = w.roofs[i];
+            if (r.x1 <= covered) continue;
+            if (r.x0 < covered) {
+                const float span = r.x1 - r.x0;
+                r.top0 += (covered - r.x0) / span * (r.top1 - r.top0);
+                r.x0 = covered;
+                r.ground0 = covered_ground;
+                r.g0 = covered_g;
+            }
+            covered = r.x1;
+            covered_ground = r.ground1;
+            covered_g = r.g1;
+            w.roofs[kept++] = r;
+        }
+        nroofs = kept;
+    }
+
+    // The walk: one pass streaming the merged profile, the candidates and both states.
+    const P2f S{0.0f, source_alt};
+    const P2f R{H, receiver_alt};
+    {
+        const float direct = sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt));
+        w.state[0].radius = 0.0f;
+        w.state[1].radius = fmaxf(1000.0f, 8.0f * direct);
+        for (int s = 0; s < 2; ++s) {
+            w.state[s].blocking = false;
+            w.state[s].has_pending = false;
+            w.state[s].have_best = false;
+            w.state[s].hull_n = 0;
+            w.state[s].best_delta = 0.0f;
+        }
+    }
+    w.profile.count = 0;
+    Emitter e{0, 0};
+    CrestGroup cg;
+    cg.open = false;
+    cg.group = 0;
+    Foliage fol;
+    fol.depth[0] = fol.depth[1] = 0.0f;
+    int r = 0;
+    float covered_to = 0.0f;
+
+    // Vertex 0: the source's ground.
+    profile_emit(w.profile, 0.0f, s0.z, s0.g);
+    foliage_sample(fol, w, S, R, H, 0.0f, s0.z, s0.forest, s0.canopy, true);
+    crest_vertex(w, cg, e, n, S, R, H, 0.0f, s0.z, failed);
+
+    int next_column = L.first_column, next_row = L.first_row, south = L.start_south;
+    const float inf = __builtin_inff();
+    // Every line crossed is one vertex: the walk ends within |columns| + |rise| + 2 steps.
+    int budget = (int)(fabsf(L.columns) + fabsf(L.rise)) + 4;
+    bool last = false;
+    while (!last) {
+        if (--budget < 0) {
+            failed |= QM_FAILED_CAPACITY;
+            break;
+        }
+        float t;
+        GroundSample v;
+        const float t_column = L.columns != 0.0f ? t_of_column(L, next_column) : inf;
+        const float t_row = L.rise != 0.0f ? t_of_row(L, next_row) : inf;
+        t = fminf(t_column, t_row);
+        if (t >= 1.0f) {
+            t = 1.0f;
+            v = r1;
+            last = true;
+        } else {
+            if (t_row <= t_column) failed |= row_vertex(T, L, next_row, t, v);
+            else failed |= column_vertex(T, L, next_column, south, t, v);
+            if (t_row <= t_column) {
+                if (L.rise > 0.0f) {
+                    south = next_row;
+                    next_row += 1;
+                } else {
+                    south = next_row - 1;
+                    next_row -= 1;
+                }
+            }
+            if (t_column <= t_row) next_column += L.column_step;
+            if (failed & QM_FAILED_NO_TERRAIN) break;
+        }
+        const float x = t * H;
+        const float z = x < platform ? fminf(v.z, source_ground) : v.z;
+        foliage_sample(fol, w, S, R, H, x, v.z, v.forest, v.canopy, false);
+        // Roofs starting before this vertex, then the vertex unless a roof covers it.
+        while (r < nroofs && w.roofs[r].x0 < x) {
+            const Roof& f = w.roofs[r];
+            profile_emit(w.profile, f.x0, f.ground0, f.g0);
+            profile_emit(w.profile, f.x0, f.top0, 0.0f);
+            profile_emit(w.profile, f.x1, f.top1, 0.0f);
+            profile_emit(w.profile, f.x1, f.ground1, f.g1);
+            covered_to = f.x1;
+            ++r;
+        }
+        if (x > covered_to) profile_emit(w.profile, x, z, v.g);
+        resolve_tops(w, e, n, x, false, failed);
+        crest_vertex(w, cg, e, n, S, R, H, x, z, failed);
+    }
+    if (failed & (QM_FAILED_NO_TERRAIN | QM_FAILED_CAPACITY)) {
+        out.failed = failed;
+        return;
+    }
+    if (cg.open) {
+        tops_before(w, e, n, S, R, H, cg.best.x, failed);
+        candidate_out(w, S, R, H, cg.best, failed);
+    }
+    resolve_tops(w, e, n, 0.0f, true, failed);
+    tops_before(w, e, n, S, R, H, inf, failed);
+
+    // The direct path (diffraction.rs:46-63) from the whole profile's prefix.
+    const RingVertex& end = w.profile.ring[(w.profile.count - 1) % RING];
+    const double Ld = (double)H;
+    const double z0 = w.profile.z0;
+    const Plane direct_plane = plane_over(0.0, Ld, z0, z0, 0.0, 0.0, end.pz, end.pxz);
+    const Geometry direct = geometry_over(direct_plane, P2d{0.0, (double)source_alt}, P2d{Ld, (double)receiver_alt});
+    const float path_ground = (float)(end.pg / Ld);
+    const float near_source = near_source_factor(direct, path_ground, source_g);
+    float att[2][BANDS];
+    for (int s = 0; s < 2; ++s)
+        state_boundary(B, w.state[s], s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
+                       near_source, source_g, s0.g, failed, att[s]);
+
+    // ray.rs:166-190: foliage, air at the slant distance, the period mix; evaluate.rs:119-152.
+    const float slant = fmaxf(sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt)), 1.0f);
+    const float azimuth = atan2f((float)(ry - sy), (float)(rx - sx));
+    float p[PERIODS];
+    for (int q = 0; q < PERIODS; ++q) p[q] = favourable_at(rcv.favourable[q], azimuth);
+    // evaluate.rs:195-203: the point divergence.
+    const float hd = fmaxf((float)length_d, src.exclusion_radius_m);
+    const float dz = receiver_alt - (s0.z + src.height_m);
+    const float d = fmaxf(sqrtf(hd * hd + dz * dz), 1.0f);
+    const float weight = 1.0f / (d * d * 12.589254117941673f);
+    float fol_db[2][BANDS];
+    for (int s = 0; s < 2; ++s) {
+        const float depth = fol.depth[s];
+        for (int b = 0; b < BANDS; ++b)
+            fol_db[s][b] = depth < 10.0f ? 0.0f
+                           : depth < 20.0f ? B.foliage_short_db[b]
+                                           : B.foliage_db_per_m[b] * fminf(depth, 200.0f);
+    }
+    for (int b = 0; b < BANDS; ++b) {
+        const float air = __expf(-rcv.alpha_db_per_km[b] * slant * 1e-3f * (LN10 / 10.0f));
+        const float eh = air * __expf(-(att[0][b] + fol_db[0][b]) * (LN10 / 10.0f));
+        const float ef = air * __expf(-(att[1][b] + fol_db[1][b]) * (LN10 / 10.0f));
+        for (int q = 0; q < PERIODS; ++q) {
+            const float power = weight * src.energy[q][b];
+            out.bands[q][b] = power * (p[q] * ef + (1.0f - p[q]) * eh);
+            out.states[0][q] += power * eh;
+            out.states[1][q] += power * ef;
+        }
+    }
+    out.failed = failed;
+}
+
+}  // namespace qm

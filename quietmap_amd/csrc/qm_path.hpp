@@ -1,0 +1,859 @@
+// One receiver-source path on the device: obstacle crossings (exact FP64), the ground under the
+// ray (lattice walk), the vertical path with roofs, both meteorological states' rubber bands and
+// CNOSSOS-EU boundary terms, foliage, air absorption and the period mix.
+//
+// Reference behaviour restated (all file:line into the reference):
+//   crossings        popup/src/obstacles/crossings.rs:13-230
+//   ground walk      popup/src/scene.rs:242-420, 532-561; tiles/src/terrain.rs:342-385
+//   vertical path    physics/src/ray.rs:125-191 (ray_terms), 201-226 (containing footprints),
+//                    255-354 (PathBuffers::fill), 360-385 (crests_of), 389-404 (roof clipping)
+//   boundary terms   physics/src/cnossos/mod.rs:123-184, mean_plane.rs:19-120, ground.rs:39-137,
+//                    rubber_band.rs:48-264, diffraction.rs:19-213
+//   foliage          physics/src/foliage.rs:22-64
+//   pair energy      popup/src/evaluate.rs:119-221, 300-330; paint/src/exact.rs:186-205
+//
+// Design (this file's own): one pass over the lattice walk per path. The wall tops and roof
+// grounds are read before the walk by bracketing each crossing between its two lattice-line
+// vertices directly, so the walk streams the merged profile (terrain plus roofs) with no lag. Prefix
+// moments of that profile (integral of z, x*z and G from the source) are kept in FP64 at every
+// vertex; each diffraction candidate carries its prefix, so a side plane over any range is a
+// difference of two prefixes without FP32 cancellation. Candidates (thinned terrain crests merged
+// with wall tops) feed both states' rubber bands as monotone chains and the unblocked-best search
+// in the same pass. Path differences (delta) are formed in FP64; band terms in FP32.
+#pragma once
+
+#include "qm_device.hpp"
+
+namespace qm {
+
+// Capacities of the per-path buffers; a path beyond any of them reports QM_FAILED_CAPACITY.
+constexpr int CROSS_CAP = 48;
+constexpr int ROOF_CAP = 24;
+constexpr int HULL_CAP = 24;
+constexpr int RING = 16;
+
+struct Scene {
+    qm_terrain terrain;
+    qm_obstacles obstacles;
+    qm_weather weather;
+};
+
+struct GroundSample {
+    float z, g, forest, canopy;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Terrain: tiles/src/terrain.rs:36-38 height code, popup/src/scene.rs:532-561 edge_sample,
+// tiles/src/terrain.rs:342-385 sample_at (fractions north and east of the south-west node).
+QM_DEV bool node_at(const qm_terrain& T, int row, int column, uint32_t& node, uint32_t& canopy) {
+    if (row < 0 || column < 0 || row >= (int)T.rows || column >= (int)T.columns) return false;
+    const size_t index = (size_t)row * T.columns + (size_t)column;
+    node = __builtin_nontemporal_load(T.nodes + index);
+    canopy = T.canopy_m[index];
+    return true;
+}
+QM_DEV bool node_valid(uint32_t n) { return (n & 0xffffu) != 0xffffu && ((n >> 16) & 0xffu) <= 100u; }
+QM_DEV float height_of(uint32_t n) { return -500.0f + (float)(n & 0xffffu) * 0.2f; }
+QM_DEV float impervious_of(uint32_t n) { return (float)((n >> 16) & 0xffu); }
+QM_DEV float forest_of(uint32_t n) { return (float)(n >> 24); }
+
+QM_DEV uint32_t edge_sample(const qm_terrain& T, int r0, int c0, int r1, int c1, float f, GroundSample& out) {
+    uint32_t a, b, ca, cb;
+    const bool ha = node_at(T, r0, c0, a, ca), hb = node_at(T, r1, c1, b, cb);
+    if (!(ha && hb)) {
+        if (ha && f <= 0.0f) {
+            b = a;
+            cb = ca;
+        } else if (hb && f >= 1.0f) {
+            a = b;
+            ca = cb;
+        } else {
+            return QM_FAILED_NO_TERRAIN;
+        }
+    }
+    const uint32_t nearer = f < 0.5f ? a : b;
+    const uint32_t nearer_canopy = f < 0.5f ? ca : cb;
+    if ((nearer >> 24) > 100u || !node_valid(a) || !node_valid(b)) return QM_FAILED_NO_TERRAIN;
+    const float za = height_of(a), zb = height_of(b);
+    const float ia = impervious_of(a), ib = impervious_of(b);
+    out.z = za + f * (zb - za);
+    out.g = 1.0f - (ia + f * (ib - ia)) * 0.01f;
+    out.forest = forest_of(nearer) * 0.01f;
+    out.canopy = (float)nearer_canopy;
+    return 0;
+}
+
+// Bilinear ground at a point given in lattice units (column, row).
+QM_DEV uint32_t ground_at(const qm_terrain& T, double cp, double rp, GroundSample& out) {
+    const double column = floor(cp), row = floor(rp);
+    const float cf = (float)(cp - column), rf = (float)(rp - row);
+    const int c = (int)column, r = (int)row;
+    const int nc = c + (cf > 0.0f ? 1 : 0), nr = r + (rf > 0.0f ? 1 : 0);
+    uint32_t n[4], k[4];
+    if (!node_at(T, r, c, n[0], k[0]) || !node_at(T, r, nc, n[1], k[1]) || !node_at(T, nr, c, n[2], k[2]) ||
+        !node_at(T, nr, nc, n[3], k[3]))
+        return QM_FAILED_NO_TERRAIN;
+    const int nearest = (rf >= 0.5f ? 2 : 0) + (cf >= 0.5f ? 1 : 0);
+    if ((n[nearest] >> 24) > 100u) return QM_FAILED_NO_TERRAIN;
+    for (int i = 0; i < 4; ++i)
+        if (!node_valid(n[i])) return QM_FAILED_NO_TERRAIN;
+    const float w[4] = {(1.0f - rf) * (1.0f - cf), (1.0f - rf) * cf, rf * (1.0f - cf), rf * cf};
+    float h = 0.0f, imp = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        h += w[i] * height_of(n[i]);
+        imp += w[i] * impervious_of(n[i]);
+    }
+    out.z = h;
+    out.g = 1.0f - imp * 0.01f;
+    out.forest = forest_of(n[nearest]) * 0.01f;
+    out.canopy = (float)k[nearest];
+    return 0;
+}
+
+// The ray's lattice geometry: position of the source (ca, ra) in lattice units, the offsets to
+// the receiver, and the first lines strictly ahead of the source.
+struct Lattice {
+    float ca, ra, columns, rise;
+    int first_column, column_step, first_row, row_step, start_south;
+};
+
+QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, double ry) {
+    Lattice L;
+    const double inv = 1.0 / T.spacing_m;
+    const double ca = (sx - T.origin_x_m) * inv, cb = (rx - T.origin_x_m) * inv;
+    const double ra = (sy - T.origin_y_m) * inv, rb = (ry - T.origin_y_m) * inv;
+    L.ca = (float)ca;
+    L.ra = (float)ra;
+    L.columns = (float)(cb - ca);
+    L.rise = (float)(rb - ra);
+    if (L.columns > 0.0f) {
+        L.first_column = (int)floorf(L.ca) + 1;
+        L.column_step = 1;
+    } else {
+        L.first_column = (int)ceilf(L.ca) - 1;
+        L.column_step = -1;
+    }
+    if (L.rise > 0.0f) {
+        L.first_row = (int)floorf(L.ra) + 1;
+        L.row_step = 1;
+        L.start_south = (int)floorf(L.ra);
+    } else if (L.rise < 0.0f) {
+        L.first_row = (int)ceilf(L.ra) - 1;
+        L.row_step = -1;
+        L.start_south = (int)ceilf(L.ra) - 1;
+    } else {
+        L.first_row = 0;
+        L.row_step = 0;
+        L.start_south = (int)floorf(L.ra);
+    }
+    return L;
+}
+
+QM_DEV float t_of_column(const Lattice& L, int c) { return ((float)c - L.ca) / L.columns; }
+QM_DEV float t_of_row(const Lattice& L, int k) { return ((float)k - L.ra) / L.rise; }
+
+// Sample on row line k at fraction t.
+QM_DEV uint32_t row_vertex(const qm_terrain& T, const Lattice& L, int k, float t, GroundSample& out) {
+    const float c = L.ca + t * L.columns;
+    const float west = floorf(c);
+    return edge_sample(T, k, (int)west, k, (int)west + 1, c - west, out);
+}
+// Sample on column line c at fraction t, the cell's south row given.
+QM_DEV uint32_t column_vertex(const qm_terrain& T, const Lattice& L, int c, int south, float t, GroundSample& out) {
+    const float r = L.ra + t * L.rise;
+    return edge_sample(T, south, c, south + 1, c, clampf(r - (float)south, 0.0f, 1.0f), out);
+}
+
+// The walk's terrain vertex nearest before (or after) fraction t, directly: popup/src/scene.rs's
+// lattice walk visits every line strictly between the ends; this finds the last line at or before
+// t (`after` false) or the first strictly after t (`after` true) of each family and reads it.
+// Returns the vertex fraction (0 or 1 for the ends, which the caller samples).
+QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, float length, bool after,
+                               float& t_out, GroundSample& out, bool& is_end) {
+    const float x = t * length;
+    float best_t = after ? 1.0f : 0.0f;
+    int kind = 0;  // 0 end, 1 column, 2 row
+    int line = 0;
+    if (L.columns != 0.0f) {
+        const float pos = L.ca + t * L.columns;
+        int c = L.column_step > 0 ? (int)floorf(pos) : (int)ceilf(pos);
+        // Adjust to the last line at or before x (by the products the walk's vertices use).
+        for (int k = 0; k < 2; ++k)
+            if (t_of_column(L, c) * length > x) c -= L.column_step;
+        for (int k = 0; k < 2; ++k)
+            if (t_of_column(L, c + L.column_step) * length <= x) c += L.column_step;
+        if (after) c += L.column_step;
+        const bool started = L.column_step > 0 ? c >= L.first_column : c <= L.first_column;
+        if (started) {
+            const float tc = t_of_column(L, c);
+            if (tc < 1.0f && tc > 0.0f) {
+                if (after ? tc < best_t : tc > best_t) {
+                    best_t = tc;
+                    kind = 1;
+                    line = c;
+                }
+            }
+        }
+    }
+    if (L.rise != 0.0f) {
+        const float pos = L.ra + t * L.rise;
+        int k = L.row_step > 0 ? (int)floorf(pos) : (int)ceilf(pos);
+        for (int j = 0; j < 2; ++j)
+            if (t_of_row(L, k) * length > x) k -= L.row_step;
+        for (int j = 0; j < 2; ++j)
+            if (t_of_row(L, k + L.row_step) * length <= x) k += L.row_step;
+        if (after) k += L.row_step;
+        const bool started = L.row_step > 0 ? k >= L.first_row : k <= L.first_row;
+        if (started) {
+            const float tr = t_of_row(L, k);
+            if (tr < 1.0f && tr > 0.0f) {
+                // At a tie the row line's reading (the walk's rule).
+                if (after ? tr <= best_t : tr >= best_t) {
+                    best_t = tr;
+                    kind = 2;
+                    line = k;
+                }
+            }
+        }
+    }
+    t_out = best_t;
+    is_end = kind == 0;
+    if (kind == 2) return row_vertex(T, L, line, best_t, out);
+    if (kind == 1) {
+        const float r = L.ra + best_t * L.rise;
+        const int south = L.rise >= 0.0f ? (int)floorf(r) : (int)ceilf(r) - 1;
+        return column_vertex(T, L, line, L.rise == 0.0f ? L.start_south : south, best_t, out);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Obstacles: exact crossings, FP64, with the reference's operation order and no contraction so
+// that every t rounds as the reference's f64 does.
+struct Crossing {
+    double t;
+    uint64_t footprint;
+    uint32_t outline;
+    uint32_t edge;
+    float height_m;
+    uint32_t flags;  // bit 0 building, bit 1 bounds roof, bit 2 dropped (own or containing)
+    float x, top;    // filled after the bracket reads
+    float z_at, g_at;
+    double pz, pxz, pg;  // prefix moments at x, filled by the walk
+};
+
+QM_DEV bool crossing_before(const Crossing& a, double t, uint64_t footprint, uint32_t outline, uint32_t edge) {
+    if (a.t != t) return a.t < t;
+    if (a.footprint != footprint) return a.footprint < footprint;
+    if (a.outline != outline) return a.outline < outline;
+    return a.edge < edge;
+}
+
+QM_DEV double side_of(const qm_obstacles& O, const double start[2], const double delta[2], size_t index) {
+#pragma clang fp contract(off)
+    const double vx = (double)O.vertices[2 * index], vy = (double)O.vertices[2 * index + 1];
+    return delta[0] * (vy - start[1]) - delta[1] * (vx - start[0]);
+}
+
+// popup/src/obstacles/crossings.rs:80-111
+QM_DEV bool edge_bounds_roof(const qm_obstacles& O, const double start[2], const double delta[2], size_t edge,
+                             size_t first, size_t count) {
+    const double a = side_of(O, start, delta, edge), b = side_of(O, start, delta, edge + 1);
+    if (a != 0.0 && b != 0.0) return true;
+    const size_t distinct = count - 1;
+    const size_t vertex = ((a == 0.0 ? edge : edge + 1) - first) % distinct;
+    double n0 = 0.0, n1 = 0.0;
+    for (size_t offset = 1; offset < distinct; ++offset) {
+        if (n0 == 0.0) n0 = side_of(O, start, delta, first + (vertex + distinct - offset) % distinct);
+        if (n1 == 0.0) n1 = side_of(O, start, delta, first + (vertex + offset) % distinct);
+        if (n0 != 0.0 && n1 != 0.0) break;
+    }
+    return n0 * n1 < 0.0;
+}
+
+QM_DEV double snap_steps(double steps) {
+    const double nearest = rint(steps);
+    return fabs(steps - nearest) < 1e-6 ? nearest : steps;
+}
+
+// popup/src/obstacles/crossings.rs:163-230: the cell walk, the padded piece box and the exact
+// test; every hit inserted in (t, footprint, outline, edge) order. Returns failure bits.
+QM_DEV uint32_t find_crossings(const qm_obstacles& O, double fx, double fy, double tx, double ty, Crossing* out,
+                               int& n) {
+#pragma clang fp contract(off)
+    n = 0;
+    const double start[2] = {snap_steps(O.origin_steps_x + fx / O.metres_per_step),
+                             snap_steps(O.origin_steps_y - fy / O.metres_per_step)};
+    const double end[2] = {snap_steps(O.origin_steps_x + tx / O.metres_per_step),
+                           snap_steps(O.origin_steps_y - ty / O.metres_per_step)};
+    const double delta[2] = {end[0] - start[0], end[1] - start[1]};
+    const double cs = (double)QM_CELL_STEPS;
+    int64_t cell[2] = {(int64_t)floor(start[0] / cs), (int64_t)floor(start[1] / cs)};
+    const int64_t last[2] = {(int64_t)floor(end[0] / cs), (int64_t)floor(end[1] / cs)};
+    const int step[2] = {delta[0] >= 0.0 ? 1 : -1, delta[1] >= 0.0 ? 1 : -1};
+    const double inf = __builtin_inf();
+    double t_delta[2], t_max[2];
+    for (int a = 0; a < 2; ++a) {
+        t_delta[a] = delta[a] != 0.0 ? fabs(cs / delta[a]) : inf;
+        if (delta[a] == 0.0) {
+            t_max[a] = inf;
+        } else {
+            const double boundary = (double)(cell[a] + (delta[a] >= 0.0 ? 1 : 0)) * cs;
+            t_max[a] = fabs((boundary - start[a]) / delta[a]);
+        }
+    }
+    const double pad = 1e-9 * (1.0 + fabs(delta[0]) + fabs(delta[1]));
+    double t_enter = 0.0;
+    uint32_t failed = 0;
+    for (;;) {
+        const int64_t gx = cell[0] - O.cell_x0, gy = cell[1] - O.cell_y0;
+        if (gx < 0 || gy < 0 || gx >= (int64_t)O.cells_x || gy >= (int64_t)O.cells_y) return QM_FAILED_NOT_READ;
+        const size_t index = (size_t)gy * O.cells_x + (size_t)gx;
+        const uint32_t r0 = O.cell_first_run[index], r1 = O.cell_first_run[index + 1];
+        if (r1 > r0) {
+            const double t_exit = fmin(fmin(t_max[0], t_max[1]), 1.0);
+            const double te = clampd(t_enter, 0.0, 1.0), tx2 = clampd(t_exit, 0.0, 1.0);
+            const double p0 = start[0] + delta[0] * te, p1 = start[1] + delta[1] * te;
+            const double q0 = start[0] + delta[0] * tx2, q1 = start[1] + delta[1] * tx2;
+            const int32_t low0 = (int32_t)floor(fmin(p0, q0) - pad), high0 = (int32_t)ceil(fmax(p0, q0) + pad);
+            const int32_t low1 = (int32_t)floor(fmin(p1, q1) - pad), high1 = (int32_t)ceil(fmax(p1, q1) + pad);
+            for (uint32_t ri = r0; ri < r1; ++ri) {
+                const qm_run run = O.runs[ri];
+                const qm_outline& record = O.outlines[run.outline];
+                const size_t first = (size_t)record.first_vertex + run.first_edge;
+                int32_t ax = O.vertices[2 * first], ay = O.vertices[2 * first + 1];
+                for (size_t next = first + 1; next <= first + run.edge_count; ++next) {
+                    const int32_t bx = O.vertices[2 * next], by = O.vertices[2 * next + 1];
+                    const bool meets = high0 >= min(ax, bx) && max(ax, bx) >= low0 && high1 >= min(ay, by) &&
+                                       max(ay, by) >= low1;
+                    if (meets) {
+                        const double ex = (double)bx - (double)ax, ey = (double)by - (double)ay;
+                        const double den = delta[0] * ey - delta[1] * ex;
+                        if (den != 0.0) {
+                            const double wx = (double)ax - start[0], wy = (double)ay - start[1];
+                            const double t = (wx * ey - wy * ex) / den;
+                            const double u = (wx * delta[1] - wy * delta[0]) / den;
+                            if (t > 0.0 && t < 1.0 && u >= 0.0 && u <= 1.0) {
+                                if (n >= CROSS_CAP) {
+                                    failed |= QM_FAILED_CAPACITY;
+                                } else {
+                                    const uint32_t edge = (uint32_t)(next - 1);
+                                    int k = n;
+                                    while (k > 0 && !crossing_before(out[k - 1], t, record.footprint_id, run.outline, edge)) {
+                                        out[k] = out[k - 1];
+                                        --k;
+                                    }
+                                    Crossing c;
+                                    c.t = t;
+                                    c.footprint = record.footprint_id;
+                                    c.outline = run.outline;
+                                    c.edge = edge;
+                                    c.height_m = (float)((double)record.height_dm / 10.0);
+                                    const bool building = record.kind == QM_KIND_BUILDING;
+                                    const bool roof = building && edge_bounds_roof(O, start, delta, edge,
+                                                                                   record.first_vertex,
+                                                                                   record.vertex_count);
+                                    c.flags = (building ? 1u : 0u) | (roof ? 2u : 0u);
+                                    out[k] = c;
+                                    ++n;
+                                }
+                            }
+                        }
+                    }
+                    ax = bx;
+                    ay = by;
+                }
+            }
+        }
+        if (cell[0] == last[0] && cell[1] == last[1]) break;
+        t_enter = fmin(t_max[0], t_max[1]);
+        int axis;
+        if (cell[0] == last[0]) axis = 1;
+        else if (cell[1] == last[1] || t_max[0] < t_max[1]) axis = 0;
+        else axis = 1;
+        t_max[axis] += t_delta[axis];
+        cell[axis] += step[axis];
+    }
+    // crossings.rs:34-56: one crossing per footprint within 1e-9 of the one kept before it.
+    int kept = 0;
+    for (int i = 0; i < n; ++i) {
+        const Crossing c = out[i];
+        bool merged = false;
+        for (int k = kept - 1; k >= 0; --k) {
+            if (!(c.t - out[k].t < 1e-9)) break;
+            if (out[k].footprint == c.footprint) {
+                out[k].flags |= (c.flags & 2u);
+                merged = true;
+                break;
+            }
+        }
+        if (!merged) out[kept++] = c;
+    }
+    n = kept;
+    return failed;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The vertical plane. P2 = (x, z).
+struct P2f {
+    float x, z;
+};
+struct P2d {
+    double x, z;
+};
+
+// A diffraction candidate: its point, the profile's altitude and ground factor just after it, and
+// the prefix moments of the profile from the source up to it (about (0, z0)).
+struct Cand {
+    float x, z, z_at, g_at;
+    double pz, pxz, pg;
+};
+
+// physics/src/cnossos/rubber_band.rs:182-193, FP32 and FP64 forms.
+QM_DEV float triangle_excess_f(P2f s, P2f o, P2f r) {
+    const float ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
+    const float lu = sqrtf(ux * ux + uz * uz), lv = sqrtf(vx * vx + vz * vz);
+    const float wx = ux + vx, wz = uz + vz;
+    const float lw = sqrtf(wx * wx + wz * wz);
+    const float dot = ux * vx + uz * vz;
+    if (dot <= 0.0f) return lu + lv - lw;
+    const float cross = ux * vz - uz * vx;
+    return 2.0f * cross * cross / ((lu * lv + dot) * (lu + lv + lw));
+}
+QM_DEV double triangle_excess_d(P2d s, P2d o, P2d r) {
+    const double ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
+    const double lu = sqrt(ux * ux + uz * uz), lv = sqrt(vx * vx + vz * vz);
+    const double wx = ux + vx, wz = uz + vz;
+    const double lw = sqrt(wx * wx + wz * wz);
+    const double dot = ux * vx + uz * vz;
+    if (dot <= 0.0) return lu + lv - lw;
+    const double cross = ux * vz - uz * vx;
+    return 2.0 * cross * cross / ((lu * lv + dot) * (lu + lv + lw));
+}
+QM_DEV float dist_f(P2f a, P2f b) {
+    const float dx = b.x - a.x, dz = b.z - a.z;
+    return sqrtf(dx * dx + dz * dz);
+}
+QM_DEV double dist_d(P2d a, P2d b) {
+    const double dx = b.x - a.x, dz = b.z - a.z;
+    return sqrt(dx * dx + dz * dz);
+}
+
+// rubber_band.rs:65-80 arc_excess (favourable state only; radius > 0 marks favourable).
+QM_DEV float arc_excess_f(float radius, float chord) {
+    if (radius <= 0.0f) return 0.0f;
+    const float x = chord / (2.0f * radius);
+    if (x > 1.0f / 16.0f) return 2.0f * radius * asinf(fminf(x, 1.0f)) - chord;
+    const float x2 = x * x;
+    return 2.0f * radius * x * x2 *
+           (1.0f / 6.0f + x2 * (3.0f / 40.0f + x2 * (5.0f / 112.0f + x2 * (35.0f / 1152.0f))));
+}
+QM_DEV double arc_excess_d(double radius, double chord) {
+    if (radius <= 0.0) return 0.0;
+    const double x = chord / (2.0 * radius);
+    if (x > 1.0 / 16.0) return 2.0 * radius * asin(fmin(x, 1.0)) - chord;
+    const double x2 = x * x;
+    return 2.0 * radius * x * x2 * (1.0 / 6.0 + x2 * (3.0 / 40.0 + x2 * (5.0 / 112.0 + x2 * (35.0 / 1152.0))));
+}
+
+// rubber_band.rs:86-121 path_difference with one point (FP32, the candidates' screening).
+QM_DEV float single_path_difference_f(float radius, P2f from, P2f o, P2f to) {
+    const float chords = triangle_excess_f(from, o, to);
+    const float arcs = arc_excess_f(radius, dist_f(from, o)) + arc_excess_f(radius, dist_f(o, to)) -
+                       arc_excess_f(radius, dist_f(from, to));
+    const float excess = chords + arcs;
+    const float chord = from.z + (to.z - from.z) * (o.x - from.x) / (to.x - from.x);
+    if (o.z >= chord) return excess;
+    if (radius <= 0.0f) return -excess;
+    const P2f a{o.x, chord};
+    return 2.0f * triangle_excess_f(from, a, to) - triangle_excess_f(from, o, to) +
+           2.0f * arc_excess_f(radius, dist_f(from, a)) + 2.0f * arc_excess_f(radius, dist_f(a, to)) -
+           arc_excess_f(radius, dist_f(from, o)) - arc_excess_f(radius, dist_f(o, to)) -
+           arc_excess_f(radius, dist_f(from, to));
+}
+
+// rubber_band.rs:86-121 path_difference over n points, FP64 (the delta of the boundary terms).
+QM_DEV double path_difference_d(double radius, P2d from, const P2d* points, int n, P2d to) {
+    if (n == 0) return 0.0;
+    double chords = 0.0, arcs = 0.0;
+    P2d start = from;
+    for (int i = 0; i < n; ++i) {
+        chords += triangle_excess_d(start, points[i], to);
+        arcs += arc_excess_d(radius, dist_d(start, points[i]));
+        start = points[i];
+    }
+    arcs += arc_excess_d(radius, dist_d(start, to)) - arc_excess_d(radius, dist_d(from, to));
+    const double excess = chords + arcs;
+    if (n > 1) return excess;
+    const P2d first = points[0];
+    const double chord = from.z + (to.z - from.z) * (first.x - from.x) / (to.x - from.x);
+    if (first.z >= chord) return excess;
+    if (radius <= 0.0) return -excess;
+    const P2d a{first.x, chord};
+    return 2.0 * triangle_excess_d(from, a, to) - triangle_excess_d(from, first, to) +
+           2.0 * arc_excess_d(radius, dist_d(from, a)) + 2.0 * arc_excess_d(radius, dist_d(a, to)) -
+           arc_excess_d(radius, dist_d(from, first)) - arc_excess_d(radius, dist_d(first, to)) -
+           arc_excess_d(radius, dist_d(from, to));
+}
+
+// rubber_band.rs:125-138, 349-359: the favourable ray's height above the chord from `from` to
+// `to` at horizontal distance `dx` from `from` (0 in calm air, radius 0).
+QM_DEV float sag_f(float radius, P2f from, P2f to, float dx) {
+    if (radius <= 0.0f) return 0.0f;
+    const float horizontal = to.x - from.x;
+    const float chord = dist_f(from, to);
+    const float gamma2 = radius * radius;
+    const float along_per_x = chord / horizontal;
+    const float centre_term = sqrtf(gamma2 - 0.25f * chord * chord);
+    const float along = dx * along_per_x;
+    const float offset = along - 0.5f * chord;
+    const float perpendicular = along * (chord - along) / (sqrtf(fmaxf(gamma2 - offset * offset, 0.0f)) + centre_term);
+    return perpendicular * along_per_x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mean planes (mean_plane.rs:19-120) from FP64 prefix moments.
+struct Plane {
+    double slope, intercept;
+};
+
+// The plane over [a, b] given the prefixes at both ends and the profile's altitude just after a.
+QM_DEV Plane plane_over(double a, double b, double z0, double z_at_a, double pz_a, double pxz_a, double pz_b,
+                        double pxz_b) {
+    const double length = b - a;
+    if (length < 1e-3) return Plane{0.0, z_at_a};
+    const double dz = pz_b - pz_a;
+    const double iz = dz - (z_at_a - z0) * length;
+    const double ixz = (pxz_b - pxz_a) - a * dz - (z_at_a - z0) * 0.5 * length * length;
+    const double slope = 12.0 * (ixz - 0.5 * length * iz) / (length * length * length);
+    const double intercept_relative = iz / length - 0.5 * slope * length;
+    return Plane{slope, z_at_a + intercept_relative - slope * a};
+}
+
+struct Geometry {
+    Plane plane;
+    float hs, hr;        // end heights over the plane, negative taken as 0
+    bool s_below, r_below;
+    float dp;            // projected distance
+};
+
+// mean_plane.rs:109-120
+QM_DEV Geometry geometry_over(Plane p, P2d from, P2d to) {
+    const double norm = sqrt(1.0 + p.slope * p.slope);
+    const double hs = (from.z - (p.slope * from.x + p.intercept)) / norm;
+    const double hr = (to.z - (p.slope * to.x + p.intercept)) / norm;
+    Geometry g;
+    g.plane = p;
+    g.hs = (float)fmax(hs, 0.0);
+    g.hr = (float)fmax(hr, 0.0);
+    g.s_below = hs < 0.0;
+    g.r_below = hr < 0.0;
+    g.dp = (float)(fabs((to.x - from.x) + p.slope * (to.z - from.z)) / norm);
+    return g;
+}
+
+// mean_plane.rs:91-96
+QM_DEV P2d mirror(Plane p, P2d q) {
+    const double norm_sq = 1.0 + p.slope * p.slope;
+    const double offset = (q.z - (p.slope * q.x + p.intercept)) / norm_sq;
+    return P2d{q.x + 2.0 * offset * p.slope, q.z - 2.0 * offset};
+}
+
+// ground.rs:39-51
+QM_DEV float near_source_factor(const Geometry& g, float path_g, float source_g) {
+    const float height_sum = fmaxf(g.hs + g.hr, MINIMUM_HEIGHT_SUM_M);
+    const float ratio = g.dp / (SHORT_PATH_HEIGHT_FACTOR * height_sum);
+    return ratio <= 1.0f ? path_g * ratio + source_g * (1.0f - ratio) : path_g;
+}
+
+// ground.rs:78-89
+QM_DEV float analytic_ground_db(const BandTable& B, int band, float dp, float zs, float zr, float gw13) {
+    const float k = B.wavenumber[band];
+    const float gw26 = gw13 * gw13;
+    const float w = 0.0185f * B.f_2_5[band] * gw26 / (B.f_1_5[band] * gw26 + 1.3e3f * B.f_0_75[band] * gw13 + 1.16e6f);
+    const float wd = w * dp;
+    const float cf = dp * (1.0f + 3.0f * wd * __expf(-sqrtf(wd))) / (1.0f + wd);
+    const float root = sqrtf(2.0f * cf / k);
+    const float ck = cf / k;
+    const float product = (zs * zs - root * zs + ck) * (zr * zr - root * zr + ck);
+    return -10.0f * __log10f(4.0f * k * k / (dp * dp) * product);
+}
+
+// ground.rs:105-137 A_ground of one (sub-)path in one state, per band.
+QM_DEV void ground_bands(const BandTable& B, const Geometry& g, float path, float impedance, float floor_factor,
+                         bool favourable, float out[BANDS]) {
+    const float dp = fmaxf(g.dp, 1e-6f);
+    const float zs = g.hs, zr = g.hr;
+    const float gw13 = __powf(fmaxf(impedance, 0.0f), 1.3f);
+    if (!favourable) {
+        if (path < HARD_GROUND_BELOW) {
+            for (int b = 0; b < BANDS; ++b) out[b] = -3.0f;
+            return;
+        }
+        const float floor = -3.0f * (1.0f - floor_factor);
+        for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, zs, zr, gw13), floor);
+        return;
+    }
+    const float height_sum0 = fmaxf(g.hs + g.hr, MINIMUM_HEIGHT_SUM_M);
+    const float base = -3.0f * (1.0f - floor_factor);
+    const float limit = SHORT_PATH_HEIGHT_FACTOR * height_sum0;
+    const float floor = g.dp <= limit ? base : base * (1.0f + 2.0f * (1.0f - limit / g.dp));
+    if (path < HARD_GROUND_BELOW) {
+        for (int b = 0; b < BANDS; ++b) out[b] = floor;
+        return;
+    }
+    const float height_sum = fmaxf(zs + zr, MINIMUM_HEIGHT_SUM_M);
+    const float curvature = FAVOURABLE_CURVATURE_A0_PER_M * dp * dp * 0.5f;
+    const float terrain = FAVOURABLE_TERRAIN_HEIGHT_COEFFICIENT * dp / height_sum;
+    const float rs = zs / height_sum, rr = zr / height_sum;
+    const float zsf = zs + curvature * (rs * rs) + terrain;
+    const float zrf = zr + curvature * (rr * rr) + terrain;
+    for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, zsf, zrf, gw13), floor);
+}
+
+// diffraction.rs:19-30
+QM_DEV float diffraction_db(const BandTable& B, int band, float delta, float c_second) {
+    const float x = 40.0f * c_second * delta / B.wavelength[band];
+    return x >= -2.0f ? 10.0f * __log10f(3.0f + x) : 0.0f;
+}
+// diffraction.rs:33-35
+QM_DEV float ground_split_db(float ground_db, float mirrored_db, float direct_db) {
+    return -20.0f * __log10f(1.0f + (amplitude_of(-ground_db) - 1.0f) * amplitude_of(-(mirrored_db - direct_db)));
+}
+
+// The per-state streaming rubber band and unblocked-best search (rubber_band.rs:202-264).
+struct StateStream {
+    float radius;  // 0 calm; the favourable arc's radius
+    bool blocking;
+    bool has_pending;
+    bool have_best;
+    int hull_n;    // entries in `hull` after the source
+    float best_delta;
+    Cand pending;
+    Cand best;
+    Cand hull[HULL_CAP];
+};
+
+QM_DEV void hull_push(StateStream& st, P2f S, const Cand& c, uint32_t& failed) {
+    while (st.hull_n >= 1) {
+        const P2f a = st.hull_n >= 2 ? P2f{st.hull[st.hull_n - 2].x, st.hull[st.hull_n - 2].z} : S;
+        const P2f b{st.hull[st.hull_n - 1].x, st.hull[st.hull_n - 1].z};
+        const P2f p{c.x, c.z};
+        const float ray_at_b = a.z + (p.z - a.z) * (b.x - a.x) / (p.x - a.x) + sag_f(st.radius, a, p, b.x - a.x);
+        if (b.z <= ray_at_b) --st.hull_n;
+        else break;
+    }
+    if (st.hull_n >= HULL_CAP) {
+        failed |= QM_FAILED_CAPACITY;
+        return;
+    }
+    st.hull[st.hull_n++] = c;
+}
+
+// One candidate (in order of distance) into a state's stream.
+QM_DEV void state_candidate(StateStream& st, P2f S, P2f R, const Cand& c, uint32_t& failed) {
+    const P2f p{c.x, c.z};
+    const float chord = S.z + (R.z - S.z) * (p.x - S.x) / (R.x - S.x);
+    if (p.z > chord + sag_f(st.radius, S, R, p.x - S.x)) {
+        if (st.has_pending && st.pending.x == p.x) {
+            if (p.z > st.pending.z) st.pending = c;
+        } else {
+            if (st.has_pending) hull_push(st, S, st.pending, failed);
+            st.pending = c;
+            st.has_pending = true;
+        }
+        st.blocking = true;
+        return;
+    }
+    if (st.blocking) return;  // a blocked state's unblocked-best is never read
+    const float delta = single_path_difference_f(st.radius, S, p, R);
+    if (!st.have_best || delta > st.best_delta) {
+        st.best_delta = delta;
+        st.best = c;
+        st.have_best = true;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The merged profile: a ring of its last vertices with their prefix moments.
+struct RingVertex {
+    float x, z, g;
+    float pad;
+    double pz, pxz, pg;
+};
+
+struct Profile {
+    RingVertex ring[RING];
+    int count;   // vertices emitted
+    double z0;   // altitude of the first vertex (the moments' origin)
+};
+
+QM_DEV void profile_emit(Profile& P, float x, float z, float g) {
+    RingVertex v;
+    v.x = x;
+    v.z = z;
+    v.g = g;
+    v.pad = 0.0f;
+    if (P.count == 0) {
+        P.z0 = z;
+        v.pz = v.pxz = v.pg = 0.0;
+    } else {
+        const RingVertex& q = P.ring[(P.count - 1) % RING];
+        const double u0 = q.x, u1 = x;
+        const double width = u1 - u0;
+        v.pz = q.pz;
+        v.pxz = q.pxz;
+        v.pg = q.pg;
+        if (width > 0.0) {
+            const double w0 = (double)q.z - P.z0, w1 = (double)z - P.z0;
+            v.pz += 0.5 * (w0 + w1) * width;
+            v.pxz += width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
+            v.pg += 0.5 * ((double)q.g + (double)g) * width;
+        }
+    }
+    P.ring[P.count % RING] = v;
+    ++P.count;
+}
+
+// The profile at x (upper side of a step: the last vertex at or before x, interpolated toward the
+// next) and its prefix moments up to x, for x within the ring; false where x lies before the ring
+// or beyond the last vertex emitted.
+QM_DEV bool profile_query(const Profile& P, float x, Cand& c) {
+    const int oldest = P.count > RING ? P.count - RING : 0;
+    int j = P.count - 1;
+    while (j >= oldest && P.ring[j % RING].x > x) --j;
+    if (j < oldest) return false;
+    const RingVertex& a = P.ring[j % RING];
+    c.z_at = a.z;
+    c.g_at = a.g;
+    c.pz = a.pz;
+    c.pxz = a.pxz;
+    c.pg = a.pg;
+    if (a.x == x) return true;
+    if (j == P.count - 1) return false;
+    const RingVertex& b = P.ring[(j + 1) % RING];
+    const float f = (x - a.x) / (b.x - a.x);
+    const float z = a.z + f * (b.z - a.z);
+    const float g = a.g + f * (b.g - a.g);
+    c.z_at = z;
+    c.g_at = g;
+    const double u0 = a.x, u1 = x, width = u1 - u0;
+    const double w0 = (double)a.z - P.z0, w1 = (double)z - P.z0;
+    c.pz += 0.5 * (w0 + w1) * width;
+    c.pxz += width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
+    c.pg += 0.5 * ((double)a.g + (double)g) * width;
+    return true;
+}
+
+struct Roof {
+    float x0, x1, top0, top1;
+    float ground0, g0, ground1, g1;  // terrain and G under each wall
+};
+
+// The result of one path.
+struct PathResult {
+    float bands[PERIODS][BANDS];  // received A-weighted energy per period and band (weighted)
+    float states[2][PERIODS];     // calm and favourable alone, per period
+    uint32_t failed;
+};
+
+// ---------------------------------------------------------------------------------------------
+// One state's boundary term (diffraction.rs:66-196), FP64 deltas, FP32 bands.
+QM_DEV void state_boundary(const BandTable& B, StateStream& st, bool favourable, P2f Sf, P2f Rf, double z0,
+                           double L, double pz_L, double pxz_L, double pg_L, const Geometry& direct,
+                           float path_ground, float near_source, float source_g, float g_start,
+                           uint32_t& failed, float att[BANDS]) {
+    // Whole path (Table 2.5.b: calm G'/G', favourable G/G').
+    ground_bands(B, direct, path_ground, favourable ? path_ground : near_source, near_source, favourable, att);
+    // The diffraction points.
+    if (st.blocking) {
+        if (st.has_pending) hull_push(st, Sf, st.pending, failed);
+        // The receiver closes the band.
+        Cand r;
+        r.x = Rf.x;
+        r.z = Rf.z;
+        hull_push(st, Sf, r, failed);
+        --st.hull_n;  // drop the receiver itself
+    } else if (st.have_best) {
+        st.hull[0] = st.best;
+        st.hull_n = 1;
+    } else {
+        return;  // no candidate at all
+    }
+    const int n = st.hull_n;
+    if (n <= 0) return;
+    P2d pts[HULL_CAP];
+    for (int i = 0; i < n; ++i) pts[i] = P2d{(double)st.hull[i].x, (double)st.hull[i].z};
+    const P2d S{(double)Sf.x, (double)Sf.z}, R{L, (double)Rf.z};
+    const double radius = favourable ? (double)st.radius : 0.0;
+    const double delta = path_difference_d(radius, S, pts, n, R);
+    double span = 0.0;
+    for (int i = 0; i + 1 < n; ++i) span += dist_d(pts[i], pts[i + 1]) + arc_excess_d(radius, dist_d(pts[i], pts[i + 1]));
+    const Cand& first = st.hull[0];
+    const Cand& last = st.hull[n - 1];
+    const Plane ps = plane_over(0.0, first.x, z0, z0, 0.0, 0.0, first.pz, first.pxz);
+    const Plane pr = plane_over(last.x, L, z0, last.z_at, last.pz, last.pxz, pz_L, pxz_L);
+    const Geometry gs = geometry_over(ps, S, pts[0]);
+    const Geometry gr = geometry_over(pr, pts[n - 1], R);
+    const P2d s_image = mirror(ps, S), r_image = mirror(pr, R);
+    const float d_direct = (float)delta;
+    const float d_s = (float)path_difference_d(radius, s_image, pts, n, R);
+    const float d_r = (float)path_difference_d(radius, S, pts, n, r_image);
+    const bool blocked = st.blocking;
+    const float rayleigh_star = blocked ? 0.0f : (float)path_difference_d(radius, s_image, pts, n, r_image);
+    // Which bands diffraction takes (Rayleigh, 2021/1226 point (9)(c)).
+    bool any = false;
+    bool admitted[BANDS];
+    for (int b = 0; b < BANDS; ++b) {
+        const float lambda = B.wavelength[b];
+        admitted[b] = blocked || (d_direct > -lambda / 20.0f && d_direct > lambda / 4.0f - rayleigh_star);
+        any = any || admitted[b];
+    }
+    if (!any) return;
+    const float span_f = (float)span;
+    const double ls = (double)first.x;
+    const float g_so = ls < 1e-3 ? g_start : (float)(first.pg / ls);
+    const double lr = L - (double)last.x;
+    const float g_or = lr < 1e-3 ? last.g_at : (float)((pg_L - last.pg) / lr);
+    const float near_so = near_source_factor(gs, g_so, source_g);
+    float ground_so[BANDS], ground_or[BANDS];
+    ground_bands(B, gs, g_so, favourable ? g_so : near_so, near_so, favourable, ground_so);
+    ground_bands(B, gr, g_or, g_or, g_or, favourable, ground_or);
+    for (int b = 0; b < BANDS; ++b) {
+        if (!admitted[b]) continue;
+        float c_second = 1.0f;
+        if (n > 1 && span_f > MULTIPLE_DIFFRACTION_MINIMUM_SPAN_M) {
+            const float x = 5.0f * B.wavelength[b] / span_f;
+            c_second = (1.0f + x * x) / (1.0f / 3.0f + x * x);
+        }
+        const float direct_db = diffraction_db(B, b, d_direct, c_second);
+        const float s_db = diffraction_db(B, b, d_s, c_second);
+        const float r_db = diffraction_db(B, b, d_r, c_second);
+        float sr = direct_db;
+        float split_so;
+        if (gs.s_below) {
+            sr = s_db;
+            split_so = ground_so[b];
+        } else {
+            split_so = ground_split_db(ground_so[b], s_db, direct_db);
+        }
+        if (!isfinite(split_so)) {
+            split_so = ground_so[b];
+            sr = s_db;
+        }
+        float split_or;
+        if (gr.r_below) {
+            sr = r_db;
+            split_or = ground_or[b];
+        } else {
+            split_or = ground_split_db(ground_or[b], r_db, sr);
+        }
+        if (!isfinite(split_or)) {
+            split_or = ground_or[b];
+            sr = r_db;
+        }
+        att[b] = clampf(sr, 0.0f, DIFFRACTION_CAP_DB) + split_so + split_or;
+    }
+}
+
+}  // namespace qm

@@ -1,0 +1,122 @@
+/* The scene as the painter reads it: plain C structs of pointers and sizes, shared by the host
+ * library, the HIP kernels and (as a data layout only) the FP64 oracle.
+ *
+ * Frame: planar metres, x east, y north. Three parts, each mirroring what the reference uploads
+ * for a square (paint-gpu/src/square_upload.rs:18-287):
+ *
+ *  - terrain: one regular lattice of decoded nodes, 4 bytes each as the reference's decoded
+ *    qmterr2 node (tiles/src/terrain.rs:21-27, 149-159): height code u16 (-500 m + code/5),
+ *    imperviousness u8 (G = 1 - imperviousness/100), forest cover u8, plus a canopy-height byte
+ *    plane. Row r lies at y = origin_y + r*spacing, column c at x = origin_x + c*spacing.
+ *  - obstacles: the cell index of buildings and walls in integer "steps" (y south), cells of 256
+ *    steps, per cell the runs of consecutive edges of one outline through it, outlines with their
+ *    footprint id, height (dm) and kind, vertices as exact integers (tiles/src/obstacles.rs:1-23,
+ *    popup/src/obstacles/mod.rs:1-8). Steps of a frame point: sx = origin_steps_x + x/mps,
+ *    sy = origin_steps_y - y/mps.
+ *  - weather: a lattice of the reference's 64-byte weather nodes (physics/src/weather.rs:19-27):
+ *    48 favourable percentages [period][sector], then 8 u16 air absorptions in 0.01 dB/km,
+ *    bilinear between nodes.
+ */
+#ifndef QM_SCENE_H
+#define QM_SCENE_H
+
+#include <stdint.h>
+
+#define QM_BANDS 8
+#define QM_PERIODS 3
+#define QM_SECTORS 16
+#define QM_LAYERS 6
+#define QM_WEATHER_NODE_BYTES 64
+#define QM_CELL_STEPS 256
+#define QM_RECEIVER_HEIGHT_M 4.0
+#define QM_GROUND_FROM_TERRAIN 255
+#define QM_PAIR_SUMS 9
+
+/* Per-pair failure bits (paint-gpu/kernels/common.cuh:27-31 semantics). */
+#define QM_FAILED_NOT_READ 1u   /* the ray leaves the scene's obstacle cells */
+#define QM_FAILED_NO_TERRAIN 2u /* a terrain node outside the lattice */
+#define QM_FAILED_CAPACITY 4u   /* a per-path buffer (crossings, hull, roofs) too small */
+
+#define QM_KIND_BUILDING 0u
+#define QM_KIND_BARRIER 1u
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qm_terrain {
+    double origin_x_m, origin_y_m;
+    double spacing_m;
+    uint32_t columns, rows;
+    const uint32_t* nodes;   /* rows*columns: height code | impervious << 16 | forest << 24 */
+    const uint8_t* canopy_m; /* rows*columns */
+} qm_terrain;
+
+typedef struct qm_run {
+    uint32_t outline;
+    uint16_t first_edge;
+    uint16_t edge_count;
+} qm_run;
+
+typedef struct qm_outline {
+    uint64_t footprint_id;
+    uint32_t first_vertex;  /* into the vertex array */
+    uint32_t vertex_count;  /* a ring repeats its first vertex last */
+    uint16_t height_dm;
+    uint8_t kind;           /* QM_KIND_* */
+    uint8_t pad[5];
+} qm_outline;
+
+typedef struct qm_obstacles {
+    double origin_steps_x, origin_steps_y;
+    double metres_per_step;
+    int32_t cell_x0, cell_y0; /* global cell of grid cell (0, 0) */
+    uint32_t cells_x, cells_y;
+    uint32_t runs_count, outlines_count, vertices_count, pad;
+    const uint32_t* cell_first_run; /* cells_x*cells_y + 1, row-major by cell y */
+    const qm_run* runs;
+    const qm_outline* outlines;
+    const int32_t* vertices; /* 2 per vertex: x, y steps */
+} qm_obstacles;
+
+typedef struct qm_weather {
+    double origin_x_m, origin_y_m;
+    double spacing_m;
+    uint32_t columns, rows;
+    const uint8_t* nodes; /* rows*columns*64 */
+} qm_weather;
+
+/* A point source: the candidate's place and its attribute (popup/src/candidates.rs:35-49). */
+typedef struct qm_source {
+    double x_m, y_m;
+    float height_m;
+    float platform_half_width_m;
+    float exclusion_radius_m;
+    uint8_t ground_percent; /* QM_GROUND_FROM_TERRAIN: the terrain's G under the source */
+    uint8_t layer;
+    uint8_t pad[2];
+    uint64_t footprint_id;
+    float energy[QM_PERIODS][QM_BANDS]; /* A-weighted sound energy per period and band */
+} qm_source;
+
+/* A receiver point (paint/src/exact.rs:18-34): where it stands and the building it stands in. */
+typedef struct qm_point {
+    double x_m, y_m;
+    uint64_t own_footprint;
+} qm_point;
+
+/* Receiver properties as the device computes them (paint-gpu kernels point_receivers). */
+typedef struct qm_receiver {
+    double x_m, y_m;
+    uint64_t own_footprint;
+    float altitude_m;
+    uint32_t failed;
+    float alpha_db_per_km[QM_BANDS];
+    float favourable[QM_PERIODS][QM_SECTORS];
+} qm_receiver;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

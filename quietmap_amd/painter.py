@@ -1,0 +1,185 @@
+"""The painter's host interface: a Python mirror of the reference's paint-gpu API
+(paint-gpu/src/lib.rs:14-22) over libqmpaint's C ABI (include/qm_paint.h).
+
+``Gpu(0)`` opens the card as ``Gpu::new(0)`` does (paint-gpu/src/device.rs:254); ``Gpu.upload``
+is ``DeviceSquare::upload`` (paint-gpu/src/square_upload.rs:18); ``Gpu.evaluate_pairs`` is
+``Gpu::evaluate_pairs`` (paint-gpu/src/device.rs:440), returning the 9 pair sums and failure bits
+per pair and the bands per point; ``Gpu.paint_grid`` paints a receiver grid with every source
+within a cutoff. There is no CPU fallback: without the built library or a gfx950 GPU these raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libqmpaint.so")
+
+# Every entry point include/qm_paint.h declares.
+EXPORTS = (
+    "qm_gpu_new",
+    "qm_gpu_name",
+    "qm_gpu_free",
+    "qm_square_upload",
+    "qm_square_bytes",
+    "qm_square_free",
+    "qm_evaluate_pairs",
+    "qm_paint_grid",
+    "qm_paint_grid_device",
+    "qm_crossings",
+)
+
+_lib = None
+
+
+class PainterError(RuntimeError):
+    pass
+
+
+def library():
+    """libqmpaint, loaded once; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise PainterError(f"{LIBRARY} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        lib = C.CDLL(LIBRARY)
+        vp, sz, u32, dbl, cp = C.c_void_p, C.c_size_t, C.c_uint32, C.c_double, C.c_char_p
+        lib.qm_gpu_new.argtypes = [C.c_int, C.POINTER(vp), cp, sz]
+        lib.qm_gpu_name.argtypes = [vp, cp, sz]
+        lib.qm_gpu_free.argtypes = [vp]
+        lib.qm_square_upload.argtypes = [vp, vp, vp, vp, vp, sz, dbl, C.POINTER(vp), cp, sz]
+        lib.qm_square_bytes.argtypes = [vp]
+        lib.qm_square_bytes.restype = sz
+        lib.qm_square_free.argtypes = [vp]
+        lib.qm_evaluate_pairs.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, cp, sz]
+        lib.qm_paint_grid.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, cp, sz]
+        lib.qm_paint_grid_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
+        lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
+        _lib = lib
+    return _lib
+
+
+def _check(rc, err):
+    if rc != 0:
+        raise PainterError(err.value.decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Square:
+    """A scene on the device (DeviceSquare)."""
+
+    def __init__(self, gpu, handle, scene, sources):
+        self.gpu, self.handle, self.scene, self.sources = gpu, handle, scene, sources
+
+    @property
+    def bytes(self):
+        return library().qm_square_bytes(self.handle)
+
+    def close(self):
+        if self.handle:
+            library().qm_square_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Gpu:
+    def __init__(self, ordinal: int = 0):
+        lib = library()
+        err = C.create_string_buffer(512)
+        handle = C.c_void_p()
+        _check(lib.qm_gpu_new(ordinal, C.byref(handle), err, 512), err)
+        self.handle = handle
+
+    @property
+    def name(self) -> str:
+        out = C.create_string_buffer(256)
+        library().qm_gpu_name(self.handle, out, 256)
+        return out.value.decode()
+
+    def close(self):
+        if self.handle:
+            library().qm_gpu_free(self.handle)
+            self.handle = None
+
+    def upload(self, scene: abi.Scene, sources, bucket_m: float = 250.0) -> Square:
+        sources = np.ascontiguousarray(sources, abi.SOURCE_DTYPE)
+        err = C.create_string_buffer(512)
+        handle = C.c_void_p()
+        rc = library().qm_square_upload(self.handle, C.byref(scene.terrain.c), C.byref(scene.obstacles.c),
+                                        C.byref(scene.weather.c), _ptr(sources), sources.size, bucket_m,
+                                        C.byref(handle), err, 512)
+        _check(rc, err)
+        return Square(self, handle, scene, sources)
+
+    def evaluate_pairs(self, square: Square, points, pair_point, pair_source, pair_weight=None, point_bands=True):
+        points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        pp = np.ascontiguousarray(pair_point, np.uint32)
+        ps = np.ascontiguousarray(pair_source, np.uint32)
+        pw = None if pair_weight is None else np.ascontiguousarray(pair_weight, np.float32)
+        n = pp.size
+        sums = np.zeros((n, abi.PAIR_SUMS), np.float32)
+        failed = np.zeros(n, np.uint32)
+        bands = np.zeros((points.size, abi.LAYERS, abi.PERIODS, abi.BANDS), np.float32) if point_bands else None
+        err = C.create_string_buffer(512)
+        rc = library().qm_evaluate_pairs(self.handle, square.handle, _ptr(points), points.size, _ptr(pp), _ptr(ps),
+                                         _ptr(pw), n, _ptr(sums), _ptr(failed), _ptr(bands), err, 512)
+        _check(rc, err)
+        return dict(sums=sums, failed=failed, point_bands=bands)
+
+    def paint_grid(self, square: Square, points, nx: int, ny: int, cutoff_m: float):
+        points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        assert points.size == nx * ny
+        energy = np.zeros((points.size, abi.PERIODS), np.float64)
+        paths = np.zeros(points.size, np.uint32)
+        failures = np.zeros(points.size, np.uint32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_paint_grid(self.handle, square.handle, _ptr(points), nx, ny, cutoff_m, _ptr(energy),
+                                     _ptr(paths), _ptr(failures), err, 512)
+        _check(rc, err)
+        return dict(period_energy=energy, paths=paths, failures=failures)
+
+    def paint_grid_device(self, square: Square, d_points: int, nx: int, ny: int, cutoff_m: float, d_energy: int,
+                          d_paths: int, d_failures: int, stream: int = 0):
+        """Device pointers in and out (inputs resident in HBM); enqueued on `stream`."""
+        err = C.create_string_buffer(512)
+        rc = library().qm_paint_grid_device(self.handle, square.handle, d_points, nx, ny, cutoff_m, d_energy,
+                                            d_paths, d_failures, stream or None, err, 512)
+        _check(rc, err)
+
+    def crossings(self, square: Square, segments, capacity: int = 48):
+        seg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)
+        n = seg.shape[0]
+        t = np.zeros((n, capacity), np.float64)
+        fp = np.zeros((n, capacity), np.uint64)
+        ol = np.zeros((n, capacity), np.uint32)
+        ed = np.zeros((n, capacity), np.uint32)
+        fl = np.zeros((n, capacity), np.uint32)
+        count = np.zeros(n, np.uint32)
+        failed = np.zeros(n, np.uint32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_crossings(self.handle, square.handle, _ptr(seg), n, capacity, _ptr(t), _ptr(fp), _ptr(ol),
+                                    _ptr(ed), _ptr(fl), _ptr(count), _ptr(failed), err, 512)
+        _check(rc, err)
+        return dict(t=t, footprint=fp, outline=ol, edge=ed, flags=fl, count=count, failed=failed)
+
+
+def lden_db(period_energy):
+    """Lden of per-period energies (physics/src/bands.rs:59-66), dB; -inf for silence."""
+    e = np.asarray(period_energy, np.float64)
+    hours = np.array([12.0, 4.0, 8.0])
+    penalty = 10.0 ** (np.array([0.0, 5.0, 10.0]) / 10.0)
+    total = (e * hours * penalty).sum(axis=-1) / 24.0
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(total)

@@ -1,0 +1,288 @@
+"""Seeded synthetic scenes of the painter's benchmark configurations, emitted directly in the scene
+ABI of ``csrc/qm_scene.h``: terrain lattice, obstacle cell index, weather lattice, point sources and
+receiver points.
+
+- ``road_scene`` (configuration "plain CNOSSOS road scene"): 256x256 receivers at 10 m, 4 m above
+  flat ground with G per 10 m node from {0, 0.5, 1}; 200 random road polylines split into 2,000
+  point sources at 0.05 m with band levels N(80, 6) dB plus the A-weighting shape; 500 rotated
+  rectangles (sides U[5, 40] m, heights U[3, 30] m); p_fav = 0.5; ISO 9613-1 air at 15 C / 70 %.
+- ``tile_scene`` (configuration "plain exact painter at tile scale"): receivers at 10 m over a
+  diamond-square DEM (30 m lattice, 200 m relief), point sources (road 0.05 m / rail 0.5 m) with
+  band levels N(85, 8) dB and day/evening/night offsets, polygon buildings with 4-12 edges and
+  heights U[3, 60] m, 2 km propagation cutoff. Its full size is 4096x4096 receivers, 200,000
+  sources and 50,000 buildings; ``scale`` shrinks the area (and the counts with it) for tests.
+
+The generators use numpy's PCG64 with the configuration's seed; they are not the reference's
+SplitMix64 streams (the reference publishes no such scenes), so the scenes are this project's
+own, fixed by seed.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import abi
+
+A_WEIGHTING_DB = np.array([-26.2, -16.1, -8.6, -3.2, 0.0, 1.2, 1.0, -1.1])
+# ISO 9613-1 at 15 C, 70 %, 101.325 kPa per octave band (dB/km), as physics/src/atmosphere.rs
+# computes it; coded to 0.01 dB/km in the weather node as the reference's table stores it.
+ALPHA_15C_70 = np.array([0.104927, 0.380965, 1.131497, 2.362992, 4.07924, 8.748437, 26.385662, 93.713703])
+METRES_PER_STEP = 0.2
+
+
+def _weather(origin, extent, percent_of, alpha=ALPHA_15C_70, spacing=1000.0):
+    columns = int(math.ceil(extent[0] / spacing)) + 2
+    rows = int(math.ceil(extent[1] / spacing)) + 2
+    nodes = np.zeros((rows, columns, abi.WEATHER_NODE_BYTES), np.uint8)
+    for r in range(rows):
+        for c in range(columns):
+            nodes[r, c] = abi.Weather.node(percent_of(r, c), alpha)
+    return abi.Weather(origin[0] - spacing / 2, origin[1] - spacing / 2, spacing, nodes)
+
+
+def _obstacles(rings, heights, kinds, footprints, origin_m, extent_m, metres_per_step=METRES_PER_STEP):
+    """The cell index of outlines given in metres: vertices snapped to integer steps (y south), each
+    ring closed by repeating its first vertex, every edge listed in every cell its step box
+    (padded by one step) touches, consecutive edges of one outline in one cell joined as a run."""
+    origin_steps = (10_000.0, 10_000.0 + extent_m[1] / metres_per_step + 5_000.0)
+    cs = abi.CELL_STEPS
+    # Grid covering the area plus a margin of cells.
+    low_steps = (origin_steps[0] + (origin_m[0] - 2000.0) / metres_per_step,
+                 origin_steps[1] - (origin_m[1] + extent_m[1] + 2000.0) / metres_per_step)
+    high_steps = (origin_steps[0] + (origin_m[0] + extent_m[0] + 2000.0) / metres_per_step,
+                  origin_steps[1] - (origin_m[1] - 2000.0) / metres_per_step)
+    cell_x0, cell_y0 = int(math.floor(low_steps[0] / cs)), int(math.floor(low_steps[1] / cs))
+    cells_x = int(math.floor(high_steps[0] / cs)) - cell_x0 + 1
+    cells_y = int(math.floor(high_steps[1] / cs)) - cell_y0 + 1
+    vertices = []
+    outlines = np.zeros(len(rings), abi.OUTLINE_DTYPE)
+    per_cell = {}
+    first = 0
+    for o, ring in enumerate(rings):
+        ring = np.asarray(ring, np.float64)
+        sx = np.round(origin_steps[0] + ring[:, 0] / metres_per_step).astype(np.int64)
+        sy = np.round(origin_steps[1] - ring[:, 1] / metres_per_step).astype(np.int64)
+        if kinds[o] == abi.KIND_BUILDING:
+            sx, sy = np.append(sx, sx[0]), np.append(sy, sy[0])
+        pts = np.stack([sx, sy], 1)
+        vertices.append(pts)
+        outlines[o]["footprint_id"] = footprints[o]
+        outlines[o]["first_vertex"] = first
+        outlines[o]["vertex_count"] = len(pts)
+        outlines[o]["height_dm"] = int(round(heights[o] * 10.0))
+        outlines[o]["kind"] = kinds[o]
+        for e in range(len(pts) - 1):
+            (ax, ay), (bx, by) = pts[e], pts[e + 1]
+            c0 = (min(ax, bx) - 1) // cs - cell_x0, (min(ay, by) - 1) // cs - cell_y0
+            c1 = (max(ax, bx) + 1) // cs - cell_x0, (max(ay, by) + 1) // cs - cell_y0
+            for cy in range(c0[1], c1[1] + 1):
+                for cx in range(c0[0], c1[0] + 1):
+                    per_cell.setdefault(cy * cells_x + cx, []).append((o, e))
+        first += len(pts)
+    runs = []
+    cell_first_run = np.zeros(cells_x * cells_y + 1, np.uint32)
+    for index in range(cells_x * cells_y):
+        cell_first_run[index] = len(runs)
+        edges = per_cell.get(index)
+        if not edges:
+            continue
+        edges.sort()
+        start = edges[0]
+        count = 1
+        for prev, cur in zip(edges, edges[1:]):
+            if cur[0] == prev[0] and cur[1] == prev[1] + 1:
+                count += 1
+            else:
+                runs.append((start[0], start[1], count))
+                start, count = cur, 1
+        runs.append((start[0], start[1], count))
+    cell_first_run[-1] = len(runs)
+    runs_arr = np.zeros(len(runs), abi.RUN_DTYPE)
+    if runs:
+        r = np.array(runs, np.int64)
+        runs_arr["outline"], runs_arr["first_edge"], runs_arr["edge_count"] = r[:, 0], r[:, 1], r[:, 2]
+    verts = np.concatenate(vertices) if vertices else np.zeros((0, 2), np.int64)
+    assert verts.size == 0 or (np.abs(verts).max() < 2**31)
+    return abi.Obstacles(origin_steps, metres_per_step, (cell_x0, cell_y0), (cells_x, cells_y), cell_first_run,
+                         runs_arr, outlines, verts.astype(np.int32))
+
+
+def _inside(ring, x, y):
+    """Even-odd point-in-polygon of points (x, y) against one ring (metres)."""
+    inside = np.zeros(x.shape, bool)
+    n = len(ring)
+    for i in range(n):
+        ax, ay = ring[i]
+        bx, by = ring[(i + 1) % n]
+        crosses = (ay > y) != (by > y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xi = ax + (y - ay) * (bx - ax) / (by - ay)
+        inside ^= crosses & (x < xi)
+    return inside
+
+
+def _own_footprints(points, rings, footprints, spacing, origin, shape):
+    """A receiver inside a building stands in it (paint/src/exact.rs:36-41): its footprint id."""
+    nx, ny = shape
+    own = points["own_footprint"].reshape(ny, nx)
+    for ring, fid in zip(rings, footprints):
+        ring = np.asarray(ring)
+        i0 = max(int(math.floor((ring[:, 0].min() - origin[0]) / spacing)), 0)
+        i1 = min(int(math.ceil((ring[:, 0].max() - origin[0]) / spacing)), nx - 1)
+        j0 = max(int(math.floor((ring[:, 1].min() - origin[1]) / spacing)), 0)
+        j1 = min(int(math.ceil((ring[:, 1].max() - origin[1]) / spacing)), ny - 1)
+        if i1 < i0 or j1 < j0:
+            continue
+        jj, ii = np.mgrid[j0 : j1 + 1, i0 : i1 + 1]
+        x, y = origin[0] + ii * spacing, origin[1] + jj * spacing
+        hit = _inside(ring, x, y) & (own[j0 : j1 + 1, i0 : i1 + 1] == 0)
+        own[j0 : j1 + 1, i0 : i1 + 1][hit] = fid
+    points["own_footprint"] = own.reshape(-1)
+
+
+def receiver_grid(nx, ny, spacing, origin):
+    """Receivers at the centres of a grid of cells (no receiver on a lattice line)."""
+    points = np.zeros(nx * ny, abi.POINT_DTYPE)
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    points["x_m"] = origin[0] + ii.reshape(-1) * spacing
+    points["y_m"] = origin[1] + jj.reshape(-1) * spacing
+    return points
+
+
+def _polyline_points(rng, count_lines, per_line, extent, origin):
+    out = []
+    for _ in range(count_lines):
+        k = rng.integers(2, 6)
+        verts = np.stack([origin[0] + rng.uniform(0, extent[0], k), origin[1] + rng.uniform(0, extent[1], k)], 1)
+        seg = np.linalg.norm(np.diff(verts, axis=0), axis=1)
+        total = seg.sum()
+        cum = np.concatenate([[0.0], np.cumsum(seg)])
+        for s in (np.arange(per_line) + 0.5) / per_line * total:
+            i = min(int(np.searchsorted(cum, s, side="right")) - 1, k - 2)
+            f = (s - cum[i]) / seg[i] if seg[i] > 0 else 0.0
+            out.append(verts[i] + f * (verts[i + 1] - verts[i]))
+    return np.array(out)
+
+
+class SyntheticScene:
+    """A scene with its sources, receivers and cutoff."""
+
+    def __init__(self, name, scene, sources, points, grid_shape, cutoff_m, rings=None):
+        self.name, self.scene, self.sources, self.points = name, scene, sources, points
+        self.grid_shape, self.cutoff_m, self.rings = grid_shape, cutoff_m, rings
+
+
+def road_scene(seed=1, receivers=256, spacing=10.0, lines=200, per_line=10, buildings=500):
+    """Configuration "plain CNOSSOS road scene": see the module docstring."""
+    rng = np.random.default_rng(seed)
+    extent = receivers * spacing
+    origin = (0.0, 0.0)
+    # Terrain: 10 m lattice from -200 m to extent + 200 m, flat at 0 m, G per node {0, 0.5, 1}.
+    n = int(round((extent + 400.0) / 10.0)) + 1
+    impervious = rng.choice(np.array([100, 50, 0]), size=(n, n))
+    terrain = abi.Terrain(-200.0, -200.0, 10.0, abi.Terrain.pack(np.zeros((n, n)), impervious))
+    # Buildings: rotated rectangles.
+    centres = rng.uniform(0, extent, (buildings, 2))
+    sides = rng.uniform(5.0, 40.0, (buildings, 2))
+    angles = rng.uniform(0, math.pi, buildings)
+    heights = rng.uniform(3.0, 30.0, buildings)
+    rings = []
+    for (cx, cy), (a, b), th in zip(centres, sides, angles):
+        c, s = math.cos(th), math.sin(th)
+        corners = [(-a / 2, -b / 2), (a / 2, -b / 2), (a / 2, b / 2), (-a / 2, b / 2)]
+        rings.append([(cx + c * u - s * v, cy + s * u + c * v) for u, v in corners])
+    footprints = np.arange(1, buildings + 1, dtype=np.uint64)
+    obstacles = _obstacles(rings, heights, [abi.KIND_BUILDING] * buildings, footprints, origin, (extent, extent))
+    weather = _weather((-1000.0, -1000.0), (extent + 2000.0, extent + 2000.0),
+                       lambda r, c: np.full((3, 16), 50))
+    # Sources: 200 polylines x 10 points at 0.05 m, band levels N(80, 6) + A-weighting shape.
+    xy = _polyline_points(rng, lines, per_line, (extent, extent), origin)
+    sources = np.zeros(len(xy), abi.SOURCE_DTYPE)
+    sources["x_m"], sources["y_m"] = xy[:, 0], xy[:, 1]
+    sources["height_m"] = 0.05
+    sources["exclusion_radius_m"] = 2.0
+    sources["ground_percent"] = abi.GROUND_FROM_TERRAIN
+    levels = rng.normal(80.0, 6.0, (len(xy), 8)) + A_WEIGHTING_DB
+    energy = 10.0 ** (levels / 10.0)
+    sources["energy"] = np.repeat(energy[:, None, :], 3, axis=1)
+    points = receiver_grid(receivers, receivers, spacing, (spacing / 2, spacing / 2))
+    _own_footprints(points, rings, footprints, spacing, (spacing / 2, spacing / 2), (receivers, receivers))
+    scene = abi.Scene(terrain, obstacles, weather)
+    return SyntheticScene("road", scene, sources, points, (receivers, receivers), float("inf"), rings)
+
+
+def _diamond_square(rng, power, relief):
+    n = 2**power + 1
+    h = np.zeros((n, n))
+    h[0, 0], h[0, -1], h[-1, 0], h[-1, -1] = rng.uniform(0, 1, 4)
+    step, scale = n - 1, 1.0
+    while step > 1:
+        half = step // 2
+        # Diamond step.
+        c = (h[0:-1:step, 0:-1:step] + h[0:-1:step, step::step] + h[step::step, 0:-1:step] + h[step::step, step::step])
+        h[half::step, half::step] = c / 4.0 + rng.uniform(-scale, scale, c.shape)
+        # Square step.
+        for (oy, ox) in ((0, half), (half, 0)):
+            ys, xs = np.mgrid[oy:n:step, ox:n:step]
+            total = np.zeros(ys.shape)
+            count = np.zeros(ys.shape)
+            for dy, dx in ((-half, 0), (half, 0), (0, -half), (0, half)):
+                yy, xx = ys + dy, xs + dx
+                ok = (yy >= 0) & (yy < n) & (xx >= 0) & (xx < n)
+                total[ok] += h[yy[ok], xx[ok]]
+                count[ok] += 1
+            h[ys, xs] = total / count + rng.uniform(-scale, scale, ys.shape)
+        step, scale = half, scale * 0.55
+    h -= h.min()
+    return h * (relief / h.max())
+
+
+def tile_scene(seed=2, scale=1.0, receivers=4096, spacing=10.0, sources=200_000, buildings=50_000, cutoff_m=2000.0):
+    """Configuration "plain exact painter at tile scale"; ``scale`` < 1 shrinks the square side by
+    that factor and the source and building counts by its square (same densities)."""
+    rng = np.random.default_rng(seed)
+    nrec = max(int(round(receivers * scale)), 8)
+    extent = nrec * spacing
+    area_ratio = (nrec / receivers) ** 2
+    n_sources = max(int(round(sources * area_ratio)), 16)
+    n_buildings = max(int(round(buildings * area_ratio)), 8)
+    origin = (0.0, 0.0)
+    # DEM: diamond-square, 30 m lattice, 200 m relief, covering the area and 600 m around it.
+    margin = 600.0
+    nodes_needed = int(math.ceil((extent + 2 * margin) / 30.0)) + 1
+    power = max(int(math.ceil(math.log2(max(nodes_needed - 1, 2)))), 2)
+    dem = _diamond_square(rng, power, 200.0)[:nodes_needed, :nodes_needed] + 100.0
+    impervious = np.clip(rng.normal(40, 30, dem.shape), 0, 100).astype(np.uint32)
+    terrain = abi.Terrain(-margin, -margin, 30.0, abi.Terrain.pack(dem, impervious))
+    # Buildings: star-shaped polygons of 4-12 edges, radii 3-20 m, heights U[3, 60] m.
+    centres = rng.uniform(0, extent, (n_buildings, 2))
+    heights = rng.uniform(3.0, 60.0, n_buildings)
+    rings = []
+    for cx, cy in centres:
+        k = int(rng.integers(4, 13))
+        ang = np.sort(rng.uniform(0, 2 * math.pi, k))
+        base = rng.uniform(5.0, 20.0)
+        rad = base * rng.uniform(0.6, 1.0, k)
+        rings.append(list(zip(cx + rad * np.cos(ang), cy + rad * np.sin(ang))))
+    footprints = np.arange(1, n_buildings + 1, dtype=np.uint64)
+    obstacles = _obstacles(rings, heights, [abi.KIND_BUILDING] * n_buildings, footprints, origin, (extent, extent))
+    weather = _weather((-margin, -margin), (extent + 2 * margin, extent + 2 * margin),
+                       lambda r, c: rng.integers(20, 80, (3, 16)))
+    # Sources: road (0.05 m) and rail (0.5 m) points along random polylines.
+    per_line = 10
+    xy = _polyline_points(rng, int(math.ceil(n_sources / per_line)), per_line, (extent, extent), origin)[:n_sources]
+    src = np.zeros(len(xy), abi.SOURCE_DTYPE)
+    src["x_m"], src["y_m"] = xy[:, 0], xy[:, 1]
+    rail = rng.uniform(0, 1, len(xy)) < 0.3
+    src["height_m"] = np.where(rail, 0.5, 0.05)
+    src["layer"] = np.where(rail, 1, 0)
+    src["exclusion_radius_m"] = 2.0
+    src["ground_percent"] = abi.GROUND_FROM_TERRAIN
+    levels = rng.normal(85.0, 8.0, (len(xy), 8)) + A_WEIGHTING_DB
+    period_offset = np.array([0.0, -2.0, -7.0])
+    src["energy"] = 10.0 ** ((levels[:, None, :] + period_offset[None, :, None]) / 10.0)
+    points = receiver_grid(nrec, nrec, spacing, (spacing / 2, spacing / 2))
+    _own_footprints(points, rings, footprints, spacing, (spacing / 2, spacing / 2), (nrec, nrec))
+    scene = abi.Scene(terrain, obstacles, weather)
+    return SyntheticScene("tile", scene, src, points, (nrec, nrec), cutoff_m, rings)

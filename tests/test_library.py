@@ -1,0 +1,35 @@
+"""The C ABI library loads without a GPU and exports every entry point include/qm_paint.h
+declares; the product path refuses to run without it (no CPU fallback)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from quietmap_amd import painter
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "qm_paint.h")).read()
+    return sorted(set(re.findall(r"\b(qm_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_python_agree():
+    assert declared() == sorted(painter.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(painter.LIBRARY):
+        pytest.skip("libqmpaint.so not built")
+    lib = ctypes.CDLL(painter.LIBRARY)
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_missing_library_raises(monkeypatch, tmp_path):
+    monkeypatch.setattr(painter, "LIBRARY", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(painter, "_lib", None)
+    with pytest.raises(painter.PainterError):
+        painter.library()
