@@ -1,0 +1,226 @@
+"""Benchmark of the exact painter on one MI355X: receiver-source paths/s and Lden cells/s.
+
+Workload (one step): paint the whole tile-scale scene — 4096x4096 receivers at 10 m (4 m above a
+diamond-square DEM of 30 m lattice and 200 m relief), 200,000 point sources (road 0.05 m / rail
+0.5 m, band levels N(85, 8) dB, day/evening/night), 50,000 polygon buildings (4-12 edges, heights
+U[3, 60] m), every source within the 2 km cutoff of each receiver — in one persistent launch.
+The scene, sources and receivers are resident in HBM before the timed region.
+
+Printed: one JSON line with the metric, the roofline of the painting kernel (algorithmic FP32
+FLOP per launch over its measured duration), the FP64 CPU oracle timed on a bounded sample of the
+same workload on the host's cores, and the painted Lden's error against that oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "receiver-source paths/s & Lden cells/s at ≤0.05 dB; surrogate vs exact, 1 GPU"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="side of the scene relative to 4096 receivers")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-loudness", action="store_true")
+    ap.add_argument("--loudness-cells", type=int, default=16 * 1024 * 1024)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (tiles shard with no exchange; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from quietmap_amd import abi, hip, scenes
+    from quietmap_amd.painter import RECEIVER_BYTES, Gpu, lden_db
+
+    t0 = time.time()
+    sc = scenes.tile_scene(scale=args.scale)
+    nx, ny = sc.grid_shape
+    n = nx * ny
+    print(f"[bench] scene {nx}x{ny} receivers, {sc.sources.size} sources, "
+          f"{sc.scene.obstacles.outlines.size} buildings, generated in {time.time() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+
+    gpu = Gpu(0)
+    square = gpu.upload(sc.scene, sc.sources)
+    stream = hip.Stream()
+    d_points = hip.DeviceBuffer.from_array(sc.points)
+    d_receivers = hip.DeviceBuffer(n * RECEIVER_BYTES)
+    d_energy = hip.DeviceBuffer(n * 3 * 8)
+    d_paths = hip.DeviceBuffer(n * 4)
+    d_failures = hip.DeviceBuffer(n * 4)
+    gpu.receivers_device(square, d_points.ptr, n, d_receivers.ptr, stream.handle)
+
+    def step():
+        gpu.paint_receivers_device(square, d_receivers.ptr, nx, ny, sc.cutoff_m, d_energy.ptr, d_paths.ptr,
+                                   d_failures.ptr, stream.handle)
+
+    for k in range(args.warmup):
+        step()
+        stream.synchronize()
+        print(f"[bench] warmup {k + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    hip.synchronize()
+    times = []
+    t_start = time.time()
+    for k in range(args.steps):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e1.seconds_since(e0))
+        print(f"[bench] step {k + 1}/{args.steps}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+    hip.synchronize()
+    wall = time.time() - t_start
+    path_counts = d_paths.to_array(np.uint32, n).astype(np.int64)
+    paths = int(path_counts.sum())
+    failures = int(d_failures.to_array(np.uint32, n).astype(np.int64).sum())
+    energy = d_energy.to_array(np.float64, (n, 3))
+    kernel_s = sum(times) / len(times)
+    value = paths / kernel_s
+    cells = n / kernel_s
+
+    # The loudness leg: ISO 532-1 on band-major planes of 28 third-octave levels per cell.
+    loud = None
+    if not args.no_loudness:
+        cells_l = args.loudness_cells
+        rng_l = np.random.default_rng(4)
+        planes = rng_l.uniform(20.0, 90.0, (28, cells_l)).astype(np.float32)
+        d_levels = hip.DeviceBuffer.from_array(planes)
+        d_sone = hip.DeviceBuffer(cells_l * 4)
+        gpu.loudness_device(d_levels.ptr, cells_l, 1, cells_l, d_sone.ptr, stream.handle)
+        stream.synchronize()
+        lt = []
+        for k in range(5):
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            gpu.loudness_device(d_levels.ptr, cells_l, 1, cells_l, d_sone.ptr, stream.handle)
+            e1.record(stream)
+            e1.synchronize()
+            lt.append(e1.seconds_since(e0))
+        ls = sum(lt) / len(lt)
+        sone = d_sone.to_array(np.float32, cells_l)
+        loud = {"cells": cells_l, "kernel_ms": ls * 1e3, "cells_per_s": cells_l / ls,
+                "roofline": {"bound": "hbm", "achieved": 116.0 * cells_l / ls / 1e9, "peak": 8000.0,
+                             "unit": "GB/s", "frac": 116.0 * cells_l / ls / 1e9 / 8000.0, "traffic": None,
+                             "kernel": "zwicker_kernel"}}
+        if not args.no_cpu:
+            from oracle import oracle
+
+            k = 20000
+            t1 = time.time()
+            want = oracle.zwicker_sone(planes[:, :k].T.astype(np.float64))
+            cpu_l = k / (time.time() - t1)
+            loud["cpu_baseline"] = {"value": cpu_l, "unit": "cells/s", "cores": 1, "kind": "port",
+                                    "sample": f"{k} cells, FP64 C++ restatement, one thread"}
+            loud["max_abs_sone_vs_oracle"] = float(np.abs(sone[:k].astype(np.float64) - want).max())
+        print(f"[bench] loudness {loud['cells_per_s']:.3e} cells/s", file=sys.stderr, flush=True)
+
+    # Algorithmic work per path from the oracle's counters on a sample of this workload's paths.
+    weights = json.load(open(os.path.join(ROOT, "quietmap_amd", "roofline.json")))
+    cpu = None
+    parity = None
+    work = None
+    if not args.no_cpu:
+        from oracle import oracle
+
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        rng = np.random.default_rng(1234)
+        probe = rng.choice(n, 8, replace=False)
+        t1 = time.time()
+        oracle.paint_points(sc.scene, sc.points[probe], sc.sources, sc.cutoff_m, threads=threads)
+        per_receiver = (time.time() - t1) / 8
+        count = int(min(max(args.cpu_seconds / max(per_receiver, 1e-6), 16), 20000))
+        sample = rng.choice(n, count, replace=False)
+        t1 = time.time()
+        ref = oracle.paint_points(sc.scene, sc.points[sample], sc.sources, sc.cutoff_m, threads=threads)
+        cpu_s = time.time() - t1
+        cpu_paths = int(ref["paths"].sum())
+        cpu = {"value": cpu_paths / cpu_s, "unit": "paths/s", "cores": threads, "kind": "port",
+               "sample": f"{count} random receivers of the workload, all {cpu_paths} paths within 2 km "
+                         f"({cpu_s:.1f} s); FP64 C++ restatement of the reference's Rust CPU path"}
+        got = lden_db(energy[sample])
+        want = lden_db(ref["period_energy"])
+        err = np.abs(got - want)
+        err = err[np.isfinite(err)]
+        parity = {"receivers": int(count), "max_abs_lden_db": float(err.max()),
+                  "p999_abs_lden_db": float(np.quantile(err, 0.999)),
+                  "paths_equal": bool(np.array_equal(path_counts[sample], ref["paths"].astype(np.int64)))}
+        # Work counters over a pair sample of the same receivers.
+        pr = rng.choice(sample, min(64, count), replace=False)
+        pairs_point, pairs_source = [], []
+        for j, i in enumerate(pr):
+            d = np.hypot(sc.sources["x_m"] - sc.points["x_m"][i], sc.sources["y_m"] - sc.points["y_m"][i])
+            near = np.flatnonzero(d <= sc.cutoff_m)
+            pairs_source.append(near)
+            pairs_point.append(np.full(near.size, j, np.uint32))
+        res = oracle.evaluate_pairs(sc.scene, sc.points[pr], sc.sources, np.concatenate(pairs_point),
+                                    np.concatenate(pairs_source), threads=threads, point_bands=False)
+        m = sum(len(x) for x in pairs_source)
+        work = {k: v / m for k, v in res["work"].items()}
+        print(f"[bench] cpu baseline {cpu['value']:.0f} paths/s on {threads} threads; work per path {work}",
+              file=sys.stderr, flush=True)
+
+    roofline = None
+    if work is not None:
+        f = weights["flop_per_path"]
+        flop_path = (f["V"] * work["vertices"] + f["C"] * work["cells"] + f["E"] * work["edges"] +
+                     f["X"] * work["crossings"] + f["K"] * work["candidates"] + f["const"])
+        b = weights["bytes_per_path"]
+        bytes_path = (b["V"] * work["vertices"] + b["C"] * work["cells"] + b["C_occ"] * work["cells_occupied"] +
+                      b["E"] * work["edges"] + b["X"] * work["crossings"] + b["const"])
+        achieved = flop_path * paths / kernel_s / 1e12
+        peak = weights["peaks"]["fp32_valu_tflops"]
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "paint_kernel_traffic.json")
+        if os.path.exists(tfile):
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": traffic, "kernel": "paint_kernel",
+                    "flop_per_path": flop_path, "compulsory_bytes_per_path": bytes_path,
+                    "kernel_ms": kernel_s * 1e3}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": kernel_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 crossings, path differences and energy sums)",
+        "data": "synthetic (seeded scene generator; no reference data)",
+        "config": {"workload": f"exact painter at tile scale: {nx}x{ny} receivers at 10 m, {sc.sources.size} "
+                               f"point sources, {sc.scene.obstacles.outlines.size} polygon buildings, 30 m DEM "
+                               f"200 m relief, {sc.cutoff_m:.0f} m cutoff",
+                   "paths_per_step": paths, "receivers": n, "failed_paths": failures, "scale": args.scale},
+        "lden_cells_per_s": cells,
+        "wall_s_timed": wall,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity_vs_fp64_oracle": parity,
+        "loudness": loud,
+        "gpu": gpu.name,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,26 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
                          double cutoff_m, double* d_period_energy, uint32_t* d_paths, uint32_t* d_failures,
                          void* stream, char* err, size_t errlen);
 
+/* The two halves of qm_paint_grid_device. Receiver properties of points on the device: ground
+ * altitude + 4 m, air absorption and favourable percentages (the reference's point_receivers,
+ * paint-gpu/kernels/paint.cu:62, over paint/src/exact.rs:95-118). */
+int qm_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, size_t n,
+                        qm_receiver* d_receivers, void* stream, char* err, size_t errlen);
+/* The grid painter over prepared receivers (nx*ny, row-major). */
+int qm_paint_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_receivers, uint32_t nx,
+                              uint32_t ny, double cutoff_m, double* d_period_energy, uint32_t* d_paths,
+                              uint32_t* d_failures, void* stream, char* err, size_t errlen);
+
+/* ISO 532-1:2017 stationary (Zwicker) loudness in sone of each cell's 28 third-octave levels
+ * (25 Hz - 12.5 kHz, dB): the reference card's loud_zwicker (paint-gpu/kernels/loudness.cuh:85-153)
+ * over physics/src/loudness.rs:148-182, the step inside Gpu::pixel_loudness
+ * (paint-gpu/src/loudness.rs:270). Host arrays, `levels` cell-major (28 floats a cell). */
+int qm_loudness(qm_gpu* gpu, const float* levels, size_t n, float* sone, char* err, size_t errlen);
+/* The same on device arrays: level (cell i, band b) at d_levels[i*cell_stride + b*band_stride];
+ * band-major planes (cell_stride 1, band_stride n) read coalesced. Enqueued on `stream`. */
+int qm_loudness_device(qm_gpu* gpu, const float* d_levels, size_t n, size_t cell_stride, size_t band_stride,
+                       float* d_sone, void* stream, char* err, size_t errlen);
+
 /* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
  * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings
  * (t, footprint, outline, edge index, flags bit 0 building / bit 1 bounds roof) and the count. */

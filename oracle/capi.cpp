@@ -254,6 +254,38 @@ int qmo_paint_points(const qm_terrain* terrain, const qm_obstacles* obstacles, c
     Scene S{*terrain, *obstacles, *weather};
     const int workers = std::max(threads, 1);
     std::vector<PairScratch> scratch(workers);
+    // Sources bucketed by the cutoff (every source when the cutoff is unbounded).
+    const bool bucketed = std::isfinite(cutoff_m) && cutoff_m > 0.0 && n_sources > 0;
+    double min_x = 0.0, min_y = 0.0;
+    int64_t bx = 1, by = 1;
+    std::vector<uint32_t> first(2, 0), order(n_sources);
+    if (bucketed) {
+        double max_x = sources[0].x_m, max_y = sources[0].y_m;
+        min_x = sources[0].x_m;
+        min_y = sources[0].y_m;
+        for (size_t j = 0; j < n_sources; ++j) {
+            min_x = std::min(min_x, sources[j].x_m);
+            min_y = std::min(min_y, sources[j].y_m);
+            max_x = std::max(max_x, sources[j].x_m);
+            max_y = std::max(max_y, sources[j].y_m);
+        }
+        bx = (int64_t)std::floor((max_x - min_x) / cutoff_m) + 1;
+        by = (int64_t)std::floor((max_y - min_y) / cutoff_m) + 1;
+        first.assign((size_t)(bx * by + 1), 0);
+        std::vector<uint32_t> bucket(n_sources);
+        for (size_t j = 0; j < n_sources; ++j) {
+            const int64_t kx = std::min((int64_t)std::floor((sources[j].x_m - min_x) / cutoff_m), bx - 1);
+            const int64_t ky = std::min((int64_t)std::floor((sources[j].y_m - min_y) / cutoff_m), by - 1);
+            bucket[j] = (uint32_t)(ky * bx + kx);
+            ++first[bucket[j] + 1];
+        }
+        for (size_t k = 0; k + 1 < first.size(); ++k) first[k + 1] += first[k];
+        std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+        for (size_t j = 0; j < n_sources; ++j) order[fill[bucket[j]]++] = (uint32_t)j;
+    } else {
+        for (size_t j = 0; j < n_sources; ++j) order[j] = (uint32_t)j;
+        first[1] = (uint32_t)n_sources;
+    }
     parallel_for(n_points, workers, [&](size_t i, int k) {
         Receiver r;
         double e[PERIODS] = {0.0, 0.0, 0.0};
@@ -261,7 +293,23 @@ int qmo_paint_points(const qm_terrain* terrain, const qm_obstacles* obstacles, c
         uint32_t failed_count = 0;
         if (receiver_of(S, points[i], r) == 0) {
             PairResult result;
-            for (size_t j = 0; j < n_sources; ++j) {
+            int64_t kx0 = 0, kx1 = 0, ky0 = 0, ky1 = 0;
+            if (bucketed) {
+                const int64_t kx = (int64_t)std::floor((r.x - min_x) / cutoff_m);
+                const int64_t ky = (int64_t)std::floor((r.y - min_y) / cutoff_m);
+                kx0 = std::max<int64_t>(kx - 1, 0);
+                kx1 = std::min<int64_t>(kx + 1, bx - 1);
+                ky0 = std::max<int64_t>(ky - 1, 0);
+                ky1 = std::min<int64_t>(ky + 1, by - 1);
+            }
+            // Sources in ascending index order, as the unbucketed loop takes them.
+            std::vector<uint32_t> near;
+            for (int64_t y = ky0; y <= ky1; ++y)
+                for (int64_t x = kx0; x <= kx1; ++x)
+                    if (x >= 0 && y >= 0 && x < bx && y < by)
+                        for (uint32_t s = first[y * bx + x]; s < first[y * bx + x + 1]; ++s) near.push_back(order[s]);
+            std::sort(near.begin(), near.end());
+            for (uint32_t j : near) {
                 const double h = std::hypot(sources[j].x_m - r.x, sources[j].y_m - r.y);
                 if (!(h <= cutoff_m)) continue;
                 ++count;

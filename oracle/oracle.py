@@ -62,6 +62,15 @@ def _declare(L):
                                      C.c_size_t, U32, U32, F32, C.c_size_t, D, U32, D, C.c_int, U64]
     L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_size_t, C.c_double, D, U64, U32, C.c_int]
+    L.qmo_zwicker.argtypes = [D, C.c_size_t, D]
+
+
+def zwicker_sone(levels):
+    """ISO 532-1 stationary loudness (sone) of third-octave levels (..., 28) in dB."""
+    lv = np.ascontiguousarray(np.asarray(levels, np.float64).reshape(-1, 28))
+    out = np.zeros(lv.shape[0])
+    lib().qmo_zwicker(_p(lv, D), lv.shape[0], _p(out, D))
+    return out
 
 
 def _p(a, t):

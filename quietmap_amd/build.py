@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBRARY = os.path.join(HERE, "libqmpaint.so")
-SOURCES = [os.path.join(CSRC, "painter.hip")]
+SOURCES = [os.path.join(CSRC, "painter.hip"), os.path.join(CSRC, "loudness.hip")]
 HEADERS = [os.path.join(CSRC, f) for f in ("qm_scene.h", "qm_device.hpp", "qm_path.hpp", "qm_eval.hpp")] + [
     os.path.join(os.path.dirname(HERE), "include", "qm_paint.h")
 ]

@@ -93,8 +93,13 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
     out[i] = r;
 }
 
+#ifndef QM_PAINT_WAVES_PER_EU
+#define QM_PAINT_WAVES_PER_EU 1
+#endif
+// Occupancy target of the path kernels (waves per SIMD): the register budget the compiler gets.
+
 // Gpu::evaluate_pairs' ray transfer (one ray per point pair): sums and point bands.
-__global__ __launch_bounds__(256) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
                                                     uint32_t n, float* sums, uint32_t* failed, float* point_bands) {
@@ -136,7 +141,7 @@ __device__ uint32_t next_tile(const GridArgs& g, int xcd) {
     return g.n_tiles;
 }
 
-__global__ __launch_bounds__(GROUP) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
+__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
     __shared__ uint32_t s_tile;
     PathScratch w;
     const int lane = threadIdx.x;
@@ -201,6 +206,11 @@ __global__ __launch_bounds__(GROUP) void paint_kernel(Scene sc, BandTable B, Sou
             g.failures[index] = fails;
         }
     }
+}
+
+__global__ void zero_kernel(uint32_t* p, uint32_t n) {
+    const uint32_t i = threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void crossings_kernel(Scene sc, const double* segments, uint32_t n, uint32_t capacity,
@@ -547,13 +557,38 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
     if (n == 0) return 0;
     if (check_device(gpu, err, errlen)) return 1;
     hipStream_t stream = (hipStream_t)stream_ptr;
-    // Receivers and queue heads live in one workspace kept for the call's stream lifetime.
     qm_receiver* d_receivers = nullptr;
-    uint32_t* d_counters = nullptr;
     QM_TRY(hipMallocAsync((void**)&d_receivers, sizeof(qm_receiver) * n, stream));
-    QM_TRY(hipMallocAsync((void**)&d_counters, sizeof(uint32_t) * XCDS, stream));
-    QM_TRY(hipMemsetAsync(d_counters, 0, sizeof(uint32_t) * XCDS, stream));
     if (prepare_receivers(square, d_points, n, d_receivers, stream, err, errlen)) return 1;
+    if (qm_paint_receivers_device(gpu, square, d_receivers, nx, ny, cutoff_m, d_period_energy, d_paths, d_failures,
+                                  stream, err, errlen))
+        return 1;
+    QM_TRY(hipFreeAsync(d_receivers, stream));
+    return 0;
+}
+
+int qm_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, size_t n,
+                        qm_receiver* d_receivers, void* stream, char* err, size_t errlen) {
+    if (!gpu || !square || (n && (!d_points || !d_receivers))) return fail(err, errlen, "null argument");
+    if (n > 0xffffffffu) return fail(err, errlen, "too many points");
+    if (check_device(gpu, err, errlen)) return 1;
+    return prepare_receivers(square, d_points, n, d_receivers, (hipStream_t)stream, err, errlen);
+}
+
+int qm_paint_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_receivers, uint32_t nx,
+                              uint32_t ny, double cutoff_m, double* d_period_energy, uint32_t* d_paths,
+                              uint32_t* d_failures, void* stream_ptr, char* err, size_t errlen) {
+    if (!gpu || !square || !d_receivers || !d_period_energy || !d_paths || !d_failures)
+        return fail(err, errlen, "null argument");
+    if (!(cutoff_m >= 0.0)) return fail(err, errlen, "cutoff must be non-negative");
+    if ((size_t)nx * ny == 0) return 0;
+    if (check_device(gpu, err, errlen)) return 1;
+    hipStream_t stream = (hipStream_t)stream_ptr;
+    // The queue heads, zeroed on the stream by a kernel (no memset node under capture).
+    uint32_t* d_counters = nullptr;
+    QM_TRY(hipMallocAsync((void**)&d_counters, sizeof(uint32_t) * XCDS, stream));
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, d_counters, (uint32_t)XCDS);
+    QM_TRY(hipGetLastError());
     GridArgs g;
     g.receivers = d_receivers;
     g.nx = nx;
@@ -575,7 +610,6 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
     groups = std::max<unsigned>(groups, 1);
     hipLaunchKernelGGL(paint_kernel, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands, square->grid, g);
     QM_TRY(hipGetLastError());
-    QM_TRY(hipFreeAsync(d_receivers, stream));
     QM_TRY(hipFreeAsync(d_counters, stream));
     return 0;
 }

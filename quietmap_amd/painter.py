@@ -30,8 +30,13 @@ EXPORTS = (
     "qm_evaluate_pairs",
     "qm_paint_grid",
     "qm_paint_grid_device",
+    "qm_receivers_device",
+    "qm_paint_receivers_device",
+    "qm_loudness",
+    "qm_loudness_device",
     "qm_crossings",
 )
+RECEIVER_BYTES = 256
 
 _lib = None
 
@@ -58,7 +63,11 @@ def library():
         lib.qm_evaluate_pairs.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, cp, sz]
         lib.qm_paint_grid.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, cp, sz]
         lib.qm_paint_grid_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
+        lib.qm_receivers_device.argtypes = [vp, vp, vp, sz, vp, vp, cp, sz]
+        lib.qm_paint_receivers_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
         lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
+        lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
+        lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
         _lib = lib
     return _lib
 
@@ -156,6 +165,35 @@ class Gpu:
         err = C.create_string_buffer(512)
         rc = library().qm_paint_grid_device(self.handle, square.handle, d_points, nx, ny, cutoff_m, d_energy,
                                             d_paths, d_failures, stream or None, err, 512)
+        _check(rc, err)
+
+    def receivers_device(self, square: Square, d_points: int, n: int, d_receivers: int, stream: int = 0):
+        """Receiver properties of `n` device points into `d_receivers` (256 bytes each)."""
+        err = C.create_string_buffer(512)
+        rc = library().qm_receivers_device(self.handle, square.handle, d_points, n, d_receivers, stream or None,
+                                           err, 512)
+        _check(rc, err)
+
+    def paint_receivers_device(self, square: Square, d_receivers: int, nx: int, ny: int, cutoff_m: float,
+                               d_energy: int, d_paths: int, d_failures: int, stream: int = 0):
+        err = C.create_string_buffer(512)
+        rc = library().qm_paint_receivers_device(self.handle, square.handle, d_receivers, nx, ny, cutoff_m,
+                                                 d_energy, d_paths, d_failures, stream or None, err, 512)
+        _check(rc, err)
+
+    def loudness(self, levels):
+        """Zwicker loudness (sone) of third-octave levels (n, 28) dB."""
+        lv = np.ascontiguousarray(np.asarray(levels, np.float32).reshape(-1, 28))
+        out = np.zeros(lv.shape[0], np.float32)
+        err = C.create_string_buffer(512)
+        _check(library().qm_loudness(self.handle, _ptr(lv), lv.shape[0], _ptr(out), err, 512), err)
+        return out
+
+    def loudness_device(self, d_levels: int, n: int, cell_stride: int, band_stride: int, d_sone: int,
+                        stream: int = 0):
+        err = C.create_string_buffer(512)
+        rc = library().qm_loudness_device(self.handle, d_levels, n, cell_stride, band_stride, d_sone,
+                                          stream or None, err, 512)
         _check(rc, err)
 
     def crossings(self, square: Square, segments, capacity: int = 48):
