@@ -17,6 +17,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../quietmap_amd/csrc/qm_aircraft.h"
 #include "../quietmap_amd/csrc/qm_scene.h"
 
 #ifdef __cplusplus
@@ -86,6 +87,19 @@ int qm_loudness(qm_gpu* gpu, const float* levels, size_t n, float* sone, char* e
  * band-major planes (cell_stride 1, band_stride n) read coalesced. Enqueued on `stream`. */
 int qm_loudness_device(qm_gpu* gpu, const float* d_levels, size_t n, size_t cell_stride, size_t band_stride,
                        float* d_sone, void* stream, char* err, size_t errlen);
+
+/* Doc 29 aircraft noise at a receiver grid from raw tracks (quietmap_amd/csrc/qm_aircraft.h): the
+ * reference card's Gpu::flight_sums (paint-gpu/src/aircraft.rs:235) evaluates the same segment
+ * physics (physics/src/doc29/segment.rs:284-381) over aircraft boxes; this entry point takes the
+ * flights' segments directly. Per receiver (`points_xy`, nx*ny x/y pairs, `receiver_altitude_m`
+ * above the tracks' datum, every segment within `reach_m` horizontally): the SEL energy per period
+ * summed over flights a day (`period_energy`, 3 doubles), the flights a day whose LAmax reaches
+ * 65 and 70 dB, and `hist` (QM_HIST_BINS counts a receiver: flights per 0.5 dB of LAmax from 40 dB).
+ * Host arrays. */
+int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_classes, const qm_track* tracks,
+                     size_t n_tracks, const float* vertices, size_t n_vertices, const double* points_xy, uint32_t nx,
+                     uint32_t ny, double receiver_altitude_m, double reach_m, double* period_energy, uint32_t* n65,
+                     uint32_t* n70, uint32_t* hist, char* err, size_t errlen);
 
 /* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
  * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings

@@ -1,0 +1,399 @@
+// TEST INFRASTRUCTURE — FP64 oracle: ECAC Doc 29 flight segments at a receiver, restated from
+// physics/src/doc29/{segment,npd,corrections,screening}.rs, and the per-receiver aircraft outputs
+// of the painter's raw-track configuration (SEL energy per period, N65/N70 event counts, an
+// integer histogram of event LAmax). Not part of the product path.
+//
+// The synthetic classes' curves are taken as given in the model atmosphere (the reference moves
+// its ANP curves there with class_increments_db, physics/src/doc29/atmosphere.rs); receivers stand
+// at the datum, where the receiver impedance adjustment (atmosphere.rs:85-90) is zero; no
+// helicopter class and no horizons (Unscreened, screening.rs:67-79).
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <limits>
+#include <vector>
+
+#include "../quietmap_amd/csrc/qm_aircraft.h"
+#include "scene.hpp"
+
+namespace qmo {
+namespace {
+
+constexpr int ND = QM_NPD_DISTANCES;
+const double NPD_DISTANCES_FT[ND] = {200.0, 400.0, 630.0, 1000.0, 2000.0, 4000.0, 6300.0, 10000.0, 16000.0, 25000.0};
+constexpr double METRES_PER_FOOT = 0.3048;
+constexpr double NPD_NEAREST_SLANT_M = 100.0 * METRES_PER_FOOT;
+constexpr double NPD_LAST_DISTANCE_M = 25000.0 * METRES_PER_FOOT;
+constexpr double TAIL_ANCHOR_M = 16000.0;
+
+double log10_npd(int k) { return std::log10(NPD_DISTANCES_FT[k]); }
+
+// physics/src/doc29/npd.rs:121-160 NpdPosition
+struct Position {
+    int interval;
+    double fraction;
+    double slant_m;
+    static Position at(double slant_m) {
+        slant_m = std::max(slant_m, NPD_NEAREST_SLANT_M);
+        const double log_d = std::log10(slant_m / METRES_PER_FOOT);
+        int interval = 0;
+        for (int k = ND - 2; k >= 1; --k)
+            if (log_d > log10_npd(k)) {
+                interval = k;
+                break;
+            }
+        const double fraction = (log_d - log10_npd(interval)) / (log10_npd(interval + 1) - log10_npd(interval));
+        return Position{interval, fraction, slant_m};
+    }
+    double linear(const double* curve) const {
+        const double lower = curve[interval];
+        return lower + fraction * (curve[interval + 1] - lower);
+    }
+    double level(const double* curve, double tail_alpha) const {
+        if (slant_m < NPD_LAST_DISTANCE_M) return linear(curve);
+        return curve[ND - 1] - 20.0 * std::log10(slant_m / NPD_LAST_DISTANCE_M) -
+               tail_alpha * (slant_m - NPD_LAST_DISTANCE_M);
+    }
+};
+
+// npd.rs:166-188 tail_absorption_db_per_m
+double tail_absorption(const double* sel) {
+    double sx = 0, sy = 0, sxx = 0, sxy = 0;
+    for (int i = 0; i < 3; ++i) {
+        const int k = ND - 3 + i;
+        const double d = NPD_DISTANCES_FT[k] * METRES_PER_FOOT;
+        const double y = sel[k] + 20.0 * std::log10(d / NPD_LAST_DISTANCE_M);
+        sx += d;
+        sy += y;
+        sxx += d * d;
+        sxy += d * y;
+    }
+    const double n = 3.0;
+    return std::max(-(n * sxy - sx * sy) / (n * sxx - sx * sx), 0.0);
+}
+
+struct Reading {
+    double sel_db, lamax_db, scaled_distance_m;
+};
+
+// npd.rs:256-330 PowerRow
+struct Row {
+    double sel[ND], lamax[ND], sel_minus_lamax[ND];
+    double tail_alpha, d0;
+    bool dipole;
+    void init(const double* s, const double* l, double v_ref_kt) {
+        bool placeholder = true;
+        for (int k = 0; k < ND; ++k) {
+            sel[k] = s[k];
+            lamax[k] = l[k];
+            sel_minus_lamax[k] = s[k] - l[k];
+            if (!(std::fabs(sel_minus_lamax[k] - 12.0) < 1e-9)) placeholder = false;
+        }
+        dipole = placeholder;
+        d0 = 2.0 / M_PI * (v_ref_kt * 1852.0 / 3600.0) * 1.0;
+        tail_alpha = tail_absorption(sel);
+    }
+    Reading read(const Position& p) const {
+        const double scaled = dipole ? p.slant_m : d0 * energy(p.linear(sel_minus_lamax));
+        return Reading{p.level(sel, tail_alpha), p.level(lamax, tail_alpha), scaled};
+    }
+    double lamax_at(const Position& p) const { return p.level(lamax, tail_alpha); }
+};
+
+struct Class {
+    Row rows[2][QM_POWER_ROWS];
+    uint32_t n_rows[2];
+    uint32_t installation;
+    double v_ref_kt;
+};
+
+struct Emission {
+    const Class* cls;
+    int departure;
+    int row;
+    double weight;
+    double speed_correction_db;
+    uint32_t installation;
+
+    // npd.rs:202-218 read_npd (power rows linear in power, thrust.rs:117-119)
+    Reading table(double slant_m) const {
+        const Position p = Position::at(slant_m);
+        const Row* rows = cls->rows[departure];
+        const Reading low = rows[row].read(p);
+        if (weight == 0.0) return low;
+        const Reading high = rows[row + 1].read(p);
+        return Reading{low.sel_db + weight * (high.sel_db - low.sel_db),
+                       low.lamax_db + weight * (high.lamax_db - low.lamax_db),
+                       low.scaled_distance_m + weight * (high.scaled_distance_m - low.scaled_distance_m)};
+    }
+    double table_lamax(const Position& p) const {
+        const Row* rows = cls->rows[departure];
+        const double low = rows[row].lamax_at(p);
+        if (weight == 0.0) return low;
+        return low + weight * (rows[row + 1].lamax_at(p) - low);
+    }
+    // segment.rs:177-190 beyond_last_db (no helicopter correction, no atmosphere shift)
+    static double beyond_last(const Position& p, double last_db, double tail_db) {
+        auto spherical = [](double s) { return 20.0 * std::log10(s / NPD_LAST_DISTANCE_M); };
+        const double alpha =
+            std::max((last_db - spherical(TAIL_ANCHOR_M) - tail_db) / (TAIL_ANCHOR_M - NPD_LAST_DISTANCE_M), 0.0);
+        return last_db - spherical(p.slant_m) - alpha * (p.slant_m - NPD_LAST_DISTANCE_M);
+    }
+    // segment.rs:126-146 SegmentEmission::read_npd
+    Reading read_npd(double slant_m) const {
+        const Position p = Position::at(slant_m);
+        const Reading reading = table(p.slant_m);
+        if (p.slant_m < NPD_LAST_DISTANCE_M) return reading;
+        const Reading last = table(NPD_LAST_DISTANCE_M), tail = table(TAIL_ANCHOR_M);
+        return Reading{beyond_last(p, last.sel_db, tail.sel_db), beyond_last(p, last.lamax_db, tail.lamax_db),
+                       reading.scaled_distance_m};
+    }
+    // segment.rs:149-162 SegmentEmission::lamax_db
+    double lamax_db(double slant_m) const {
+        const Position p = Position::at(slant_m);
+        if (p.slant_m < NPD_LAST_DISTANCE_M) return table_lamax(p);
+        const double last = table_lamax(Position::at(NPD_LAST_DISTANCE_M));
+        const double tail = table_lamax(Position::at(TAIL_ANCHOR_M));
+        return beyond_last(p, last, tail);
+    }
+};
+
+// physics/src/doc29/corrections.rs:23-29
+double speed_correction_db(double reference_speed_kt, double speed_kt) {
+    return speed_kt > 10.0 ? 10.0 * std::log10(reference_speed_kt / speed_kt) : 0.0;
+}
+
+}  // namespace
+
+// corrections.rs:36-45
+double finite_segment_energy_fraction(double along_m, double length_m, double scaled_distance_m) {
+    if (length_m < 1.0 || scaled_distance_m < 1.0) return 1.0;
+    const double a0 = -along_m / scaled_distance_m;
+    const double a1 = -(along_m - length_m) / scaled_distance_m;
+    auto e = [](double a) { return a / (1.0 + a * a) + std::atan(a); };
+    return std::max((e(a1) - e(a0)) * (1.0 / M_PI), 1e-15);
+}
+
+// corrections.rs:51-67
+double lateral_attenuation_db(double height_m, double lateral_m) {
+    const double beta_deg = std::atan(height_m / std::max(lateral_m, 0.01)) * (180.0 / M_PI);
+    if (beta_deg > 50.0) return 0.0;
+    const double gamma = lateral_m <= 914.0 ? 1.089 * (1.0 - std::exp(-0.00274 * lateral_m)) : 1.0;
+    const double lambda_beta = beta_deg < 0.0 ? 10.857 : 1.137 - 0.0229 * beta_deg + 9.72 * std::exp(-0.142 * beta_deg);
+    return gamma * lambda_beta;
+}
+
+// corrections.rs:72-88
+double installation_energy_gain(uint32_t installation, double height_m, double slant_m) {
+    double a, b, c;
+    if (installation == QM_INSTALL_WING) {
+        a = 0.0039;
+        b = 0.062;
+        c = 0.8786;
+    } else if (installation == QM_INSTALL_FUSELAGE) {
+        a = 0.1225;
+        b = 0.329;
+        c = 1.0;
+    } else {
+        return 1.0;
+    }
+    height_m = std::max(height_m, 0.0);
+    const double sin2 = height_m * height_m / std::max(slant_m * slant_m, 1e-12);
+    const double cos2 = 1.0 - sin2;
+    const double numerator = a * cos2 + sin2;
+    const double d = cos2 - sin2;
+    const double denominator = c * 4.0 * sin2 * cos2 + d * d;
+    if (denominator > 0.0 && numerator > 0.0) return std::pow(numerator, b) / denominator;
+    return 1.0;
+}
+
+// physics/src/doc29/segment.rs:200-268 closest_points
+struct Closest {
+    double along;
+    double on_line[3];
+    double on_segment[3];
+    double lateral_m, height_m, lambda_height_m, length_m;
+};
+
+Closest closest_points(const double s[3], const double e[3]) {
+    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    const double horizontal_squared = d[0] * d[0] + d[1] * d[1];
+    const double length_squared = horizontal_squared + d[2] * d[2];
+    Closest c;
+    if (horizontal_squared <= 1e-6) {
+        c.along = 0.0;
+        for (int a = 0; a < 3; ++a) c.on_line[a] = c.on_segment[a] = s[a];
+        c.lateral_m = std::hypot(s[0], s[1]);
+        c.height_m = s[2];
+        c.lambda_height_m = s[2];
+        c.length_m = std::max(std::sqrt(length_squared), 1.0);
+        return c;
+    }
+    const double along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / length_squared;
+    const double clamped = std::min(std::max(along, 0.0), 1.0);
+    for (int a = 0; a < 3; ++a) {
+        c.on_line[a] = s[a] + along * d[a];
+        c.on_segment[a] = s[a] + clamped * d[a];
+    }
+    c.along = along;
+    c.lateral_m = std::fabs(s[0] * d[1] - s[1] * d[0]) / std::sqrt(horizontal_squared);
+    const double slant_squared = c.on_line[0] * c.on_line[0] + c.on_line[1] * c.on_line[1] + c.on_line[2] * c.on_line[2];
+    c.height_m = std::copysign(std::sqrt(std::max(slant_squared - c.lateral_m * c.lateral_m, 0.0)), c.on_line[2]);
+    const double cos_climb = std::sqrt(horizontal_squared / length_squared);
+    c.lambda_height_m = (along >= 0.0 && along <= 1.0) ? c.height_m : c.on_segment[2] / cos_climb;
+    c.length_m = std::max(std::sqrt(length_squared), 1.0);
+    return c;
+}
+
+static double norm3(const double p[3]) { return std::hypot(std::hypot(p[0], p[1]), p[2]); }
+
+// segment.rs:318-350 screened_segment_sel, unscreened.
+double segment_sel_db(const Emission& em, const Closest& c) {
+    const double slant = norm3(c.on_line);
+    const Reading npd = em.read_npd(slant);
+    const double finite = 10.0 * std::log10(finite_segment_energy_fraction(c.along * c.length_m, c.length_m,
+                                                                           npd.scaled_distance_m));
+    const double lateral = lateral_attenuation_db(c.lambda_height_m, c.lateral_m);
+    const double installation = 10.0 * std::log10(installation_energy_gain(em.installation, c.height_m, slant));
+    const double free_sel = npd.sel_db + em.speed_correction_db + installation - lateral + finite;
+    return free_sel - std::max(0.0 - lateral, 0.0);  // screening.rs:56-63 with no losses
+}
+
+// segment.rs:357-381 segment_lmax_db (receiver impedance zero at the datum).
+double segment_lmax_db(const Emission& em, const Closest& c) {
+    const double distance = norm3(c.on_segment);
+    const double lamax = em.lamax_db(distance);
+    const double line_slant = norm3(c.on_line);
+    const double lateral = (c.along >= 0.0 && c.along <= 1.0)
+                               ? lateral_attenuation_db(c.height_m, c.lateral_m)
+                               : lateral_attenuation_db(c.on_segment[2], std::hypot(c.on_segment[0], c.on_segment[1]));
+    return lamax + 10.0 * std::log10(installation_energy_gain(em.installation, c.height_m, line_slant)) - lateral;
+}
+
+// The reach rule of the raw-track configuration: a segment is heard when its horizontal distance
+// from the receiver is at most the reach (the segment's 2-D closest point, squared).
+bool within_reach(const double s[2], const double e[2], double reach_m) {
+    const double dx = e[0] - s[0], dy = e[1] - s[1];
+    const double ll = dx * dx + dy * dy;
+    double t = 0.0;
+    if (ll > 0.0) t = std::min(std::max(-(s[0] * dx + s[1] * dy) / ll, 0.0), 1.0);
+    const double px = s[0] + t * dx, py = s[1] + t * dy;
+    return px * px + py * py <= reach_m * reach_m;
+}
+
+// The histogram bin of an event level, -1 below the floor.
+int hist_bin(double lmax_db) {
+    if (!(lmax_db >= QM_HIST_FLOOR_DB)) return -1;
+    const double k = std::floor((lmax_db - QM_HIST_FLOOR_DB) / QM_HIST_STEP_DB);
+    return k >= QM_HIST_BINS - 1 ? QM_HIST_BINS - 1 : (int)k;
+}
+
+}  // namespace qmo
+
+using namespace qmo;
+
+extern "C" {
+
+double qmo_finite_fraction(double along, double length, double scaled) {
+    return finite_segment_energy_fraction(along, length, scaled);
+}
+double qmo_lateral_attenuation(double height, double lateral) { return lateral_attenuation_db(height, lateral); }
+double qmo_installation_gain(uint32_t inst, double height, double slant) {
+    return installation_energy_gain(inst, height, slant);
+}
+void qmo_closest_points(const double* s, const double* e, double* out /* 12 */) {
+    const Closest c = closest_points(s, e);
+    out[0] = c.along;
+    for (int a = 0; a < 3; ++a) {
+        out[1 + a] = c.on_line[a];
+        out[4 + a] = c.on_segment[a];
+    }
+    out[7] = c.lateral_m;
+    out[8] = c.height_m;
+    out[9] = c.lambda_height_m;
+    out[10] = c.length_m;
+    out[11] = 0.0;
+}
+
+// One segment's SEL and LAmax (dB) of a track's emission, given in the receiver's frame.
+int qmo_segment_levels(const qm_aircraft_class* classes, const qm_track* track, const double* s, const double* e,
+                       double* sel_db, double* lmax_db) {
+    Class cls;
+    const qm_aircraft_class& src = classes[track->aircraft_class];
+    for (int dep = 0; dep < 2; ++dep) {
+        cls.n_rows[dep] = src.rows[dep];
+        for (uint32_t r = 0; r < src.rows[dep]; ++r) cls.rows[dep][r].init(src.sel_db[dep][r], src.lamax_db[dep][r], src.v_ref_kt);
+    }
+    Emission em{&cls, (int)track->departure, (int)track->power_row, (double)track->power_weight,
+                speed_correction_db(src.v_ref_kt, (double)track->speed_kt), src.installation};
+    const Closest c = closest_points(s, e);
+    *sel_db = segment_sel_db(em, c);
+    *lmax_db = segment_lmax_db(em, c);
+    return 0;
+}
+
+// Per receiver: SEL energy per period (sum over flights of 10^(SEL/10)), N65 and N70 (flights a
+// day whose LAmax reaches 65 / 70 dB) and the LAmax histogram (flights a day per bin).
+int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, const qm_track* tracks, size_t n_tracks,
+                        const float* vertices, const double* points /* x, y pairs */, size_t n_points,
+                        double receiver_altitude_m, double reach_m, double* energy_out /* 3 a point */,
+                        uint32_t* n65, uint32_t* n70, uint32_t* hist /* QM_HIST_BINS a point */,
+                        uint64_t* segments_heard, int threads) {
+    std::vector<Class> cls(n_classes);
+    for (size_t k = 0; k < n_classes; ++k) {
+        cls[k].installation = classes[k].installation;
+        cls[k].v_ref_kt = classes[k].v_ref_kt;
+        for (int dep = 0; dep < 2; ++dep) {
+            cls[k].n_rows[dep] = classes[k].rows[dep];
+            for (uint32_t r = 0; r < classes[k].rows[dep]; ++r)
+                cls[k].rows[dep][r].init(classes[k].sel_db[dep][r], classes[k].lamax_db[dep][r], classes[k].v_ref_kt);
+        }
+    }
+    for (size_t t = 0; t < n_tracks; ++t)
+        if (tracks[t].aircraft_class >= n_classes ||
+            tracks[t].power_row + (tracks[t].power_weight > 0.0f ? 1u : 0u) >= cls[tracks[t].aircraft_class].n_rows[tracks[t].departure])
+            return -1;
+    parallel_for(n_points, std::max(threads, 1), [&](size_t i, int) {
+        const double rx = points[2 * i], ry = points[2 * i + 1];
+        double e[3] = {0.0, 0.0, 0.0};
+        uint32_t c65 = 0, c70 = 0;
+        uint64_t heard = 0;
+        uint32_t* h = hist + i * QM_HIST_BINS;
+        for (int b = 0; b < QM_HIST_BINS; ++b) h[b] = 0;
+        for (size_t t = 0; t < n_tracks; ++t) {
+            const qm_track& tr = tracks[t];
+            const Class& c = cls[tr.aircraft_class];
+            Emission em{&c, (int)tr.departure, (int)tr.power_row, (double)tr.power_weight,
+                        speed_correction_db(c.v_ref_kt, (double)tr.speed_kt), c.installation};
+            double track_energy = 0.0;
+            double lmax = -std::numeric_limits<double>::infinity();
+            bool any = false;
+            for (uint32_t v = 0; v + 1 < tr.vertex_count; ++v) {
+                const float* a = vertices + 3 * (size_t)(tr.first_vertex + v);
+                const float* b = a + 3;
+                const double s[3] = {(double)a[0] - rx, (double)a[1] - ry, (double)a[2] - receiver_altitude_m};
+                const double en[3] = {(double)b[0] - rx, (double)b[1] - ry, (double)b[2] - receiver_altitude_m};
+                if (!within_reach(s, en, reach_m)) continue;
+                any = true;
+                ++heard;
+                const Closest cp = closest_points(s, en);
+                track_energy += energy(segment_sel_db(em, cp));
+                lmax = std::max(lmax, segment_lmax_db(em, cp));
+            }
+            if (!any) continue;
+            for (int p = 0; p < 3; ++p) e[p] += (double)tr.flights[p] * track_energy;
+            const uint32_t flights = tr.flights[0] + tr.flights[1] + tr.flights[2];
+            if (lmax >= 65.0) c65 += flights;
+            if (lmax >= 70.0) c70 += flights;
+            const int bin = hist_bin(lmax);
+            if (bin >= 0) h[bin] += flights;
+        }
+        for (int p = 0; p < 3; ++p) energy_out[3 * i + p] = e[p];
+        n65[i] = c65;
+        n70[i] = c70;
+        if (segments_heard) segments_heard[i] = heard;
+    });
+    return 0;
+}
+
+}  // extern "C"

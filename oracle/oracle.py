@@ -63,6 +63,68 @@ def _declare(L):
     L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_size_t, C.c_double, D, U64, U32, C.c_int]
     L.qmo_zwicker.argtypes = [D, C.c_size_t, D]
+    L.qmo_finite_fraction.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.qmo_finite_fraction.restype = C.c_double
+    L.qmo_lateral_attenuation.argtypes = [C.c_double, C.c_double]
+    L.qmo_lateral_attenuation.restype = C.c_double
+    L.qmo_installation_gain.argtypes = [C.c_uint32, C.c_double, C.c_double]
+    L.qmo_installation_gain.restype = C.c_double
+    L.qmo_closest_points.argtypes = [D, D, D]
+    L.qmo_segment_levels.argtypes = [C.c_void_p, C.c_void_p, D, D, D, D]
+    L.qmo_aircraft_points.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, D, C.c_size_t,
+                                      C.c_double, C.c_double, D, U32, U32, U32, U64, C.c_int]
+
+
+def finite_fraction(along, length, scaled):
+    return lib().qmo_finite_fraction(along, length, scaled)
+
+
+def lateral_attenuation(height, lateral):
+    return lib().qmo_lateral_attenuation(height, lateral)
+
+
+def installation_gain(installation, height, slant):
+    return lib().qmo_installation_gain(installation, height, slant)
+
+
+def closest_points(start, end):
+    s = np.ascontiguousarray(start, np.float64)
+    e = np.ascontiguousarray(end, np.float64)
+    out = np.zeros(12)
+    lib().qmo_closest_points(_p(s, D), _p(e, D), _p(out, D))
+    return dict(along=out[0], on_line=out[1:4], on_segment=out[4:7], lateral_m=out[7], height_m=out[8],
+                lambda_height_m=out[9], length_m=out[10])
+
+
+def segment_levels(classes, track, start, end):
+    """SEL and LAmax (dB) of one segment of `track` in the receiver's frame."""
+    classes = np.ascontiguousarray(classes, abi.AIRCRAFT_CLASS_DTYPE)
+    track = np.ascontiguousarray(np.asarray(track, abi.TRACK_DTYPE).reshape(1))
+    s = np.ascontiguousarray(start, np.float64)
+    e = np.ascontiguousarray(end, np.float64)
+    sel, lmax = np.zeros(1), np.zeros(1)
+    lib().qmo_segment_levels(classes.ctypes.data, track.ctypes.data, _p(s, D), _p(e, D), _p(sel, D), _p(lmax, D))
+    return sel[0], lmax[0]
+
+
+def aircraft_points(scene, points_xy, threads=None):
+    """Per receiver: SEL energy per period, N65, N70, LAmax histogram, segments heard."""
+    xy = np.ascontiguousarray(points_xy, np.float64).reshape(-1, 2)
+    n = xy.shape[0]
+    energy = np.zeros((n, 3))
+    n65, n70 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    hist = np.zeros((n, abi.HIST_BINS), np.uint32)
+    heard = np.zeros(n, np.uint64)
+    classes = np.ascontiguousarray(scene.classes, abi.AIRCRAFT_CLASS_DTYPE)
+    tracks = np.ascontiguousarray(scene.tracks, abi.TRACK_DTYPE)
+    verts = np.ascontiguousarray(scene.vertices, np.float32)
+    rc = lib().qmo_aircraft_points(classes.ctypes.data, classes.size, tracks.ctypes.data, tracks.size,
+                                   verts.ctypes.data, _p(xy, D), n, scene.receiver_altitude_m, scene.reach_m,
+                                   _p(energy, D), _p(n65, U32), _p(n70, U32), _p(hist, U32), _p(heard, U64),
+                                   threads or os.cpu_count() or 1)
+    if rc != 0:
+        raise ValueError("oracle: track out of its class's power rows")
+    return dict(period_energy=energy, n65=n65, n70=n70, hist=hist, segments=heard)
 
 
 def zwicker_sone(levels):

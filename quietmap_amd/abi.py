@@ -191,3 +191,35 @@ class Scene:
 
     def __init__(self, terrain: Terrain, obstacles: Obstacles, weather: Weather):
         self.terrain, self.obstacles, self.weather = terrain, obstacles, weather
+
+# Aircraft (csrc/qm_aircraft.h).
+NPD_DISTANCES = 10
+POWER_ROWS = 4
+INSTALL_WING, INSTALL_FUSELAGE, INSTALL_PROPELLER = 0, 1, 2
+HIST_BINS = 160
+HIST_FLOOR_DB = 40.0
+HIST_STEP_DB = 0.5
+AIRCRAFT_CLASS_DTYPE = np.dtype(
+    [
+        ("sel_db", "<f8", (2, POWER_ROWS, NPD_DISTANCES)),
+        ("lamax_db", "<f8", (2, POWER_ROWS, NPD_DISTANCES)),
+        ("rows", "<u4", (2,)),
+        ("installation", "<u4"),
+        ("pad", "<u4"),
+        ("v_ref_kt", "<f8"),
+    ]
+)
+TRACK_DTYPE = np.dtype(
+    [
+        ("first_vertex", "<u4"),
+        ("vertex_count", "<u4"),
+        ("aircraft_class", "<u4"),
+        ("departure", "<u4"),
+        ("power_row", "<u4"),
+        ("power_weight", "<f4"),
+        ("speed_kt", "<f4"),
+        ("flights", "<u4", (3,)),
+    ]
+)
+assert AIRCRAFT_CLASS_DTYPE.itemsize == 2 * 2 * POWER_ROWS * NPD_DISTANCES * 8 + 24
+assert TRACK_DTYPE.itemsize == 40

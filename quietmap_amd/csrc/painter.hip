@@ -22,6 +22,7 @@
 
 #include "../../include/qm_paint.h"
 #include "qm_eval.hpp"
+#include "qm_internal.hpp"
 
 using namespace qm;
 
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 }
 
 #ifndef QM_PAINT_WAVES_PER_EU
-#define QM_PAINT_WAVES_PER_EU 1
+#define QM_PAINT_WAVES_PER_EU 2
 #endif
 // Occupancy target of the path kernels (waves per SIMD): the register budget the compiler gets.
 
@@ -237,12 +238,6 @@ __global__ __launch_bounds__(256) void crossings_kernel(Scene sc, const double* 
 // =============================================================================================
 // Host side.
 
-struct qm_gpu {
-    int ordinal;
-    char name[256];
-    BandTable bands;
-    std::mutex lock;
-};
 
 struct qm_square {
     qm_gpu* gpu;

@@ -1,0 +1,54 @@
+/* Aircraft noise (ECAC Doc 29) as the painter reads it: noise classes with their NPD power rows,
+ * flight tracks as polylines of segments, and the per-receiver outputs. Plain C structs shared by
+ * the host library, the HIP kernels and (as a data layout only) the FP64 oracle.
+ *
+ * A class mirrors the reference's NPD profile and thrust rows (physics/src/doc29/npd.rs:62-96,
+ * 256-330): SEL and LAmax at the ten NPD distances for up to QM_POWER_ROWS power settings per
+ * operation (approach, departure), the NPD reference speed and the engine installation. A track is
+ * one flight path flown `flights[period]` times a day: one class, one operation, one power
+ * bracket (row and weight, physics/src/doc29/thrust.rs:104-119) and one ground speed.
+ */
+#ifndef QM_AIRCRAFT_H
+#define QM_AIRCRAFT_H
+
+#include <stdint.h>
+
+#define QM_NPD_DISTANCES 10
+#define QM_POWER_ROWS 4
+#define QM_INSTALL_WING 0u
+#define QM_INSTALL_FUSELAGE 1u
+#define QM_INSTALL_PROPELLER 2u
+/* The event-level histogram: LAmax bins of 0.5 dB from 40 dB; the last bin holds every level
+ * above; levels below 40 dB are not binned. */
+#define QM_HIST_BINS 160
+#define QM_HIST_FLOOR_DB 40.0
+#define QM_HIST_STEP_DB 0.5
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qm_aircraft_class {
+    double sel_db[2][QM_POWER_ROWS][QM_NPD_DISTANCES];   /* [departure][row][distance] */
+    double lamax_db[2][QM_POWER_ROWS][QM_NPD_DISTANCES];
+    uint32_t rows[2];                                   /* power rows per operation (1..4) */
+    uint32_t installation;                              /* QM_INSTALL_* */
+    double v_ref_kt;
+} qm_aircraft_class;
+
+typedef struct qm_track {
+    uint32_t first_vertex; /* into the vertex array: x, y, z float triples (frame metres, z above datum) */
+    uint32_t vertex_count; /* segments = vertex_count - 1 */
+    uint32_t aircraft_class;
+    uint32_t departure;    /* 1 departure NPDs, 0 approach */
+    uint32_t power_row;    /* the bracket's lower row */
+    float power_weight;    /* toward the next row, in [0, 1] */
+    float speed_kt;        /* ground speed (Delta_V) */
+    uint32_t flights[3];   /* flights a day: day, evening, night */
+} qm_track;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

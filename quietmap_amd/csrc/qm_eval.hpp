@@ -75,7 +75,7 @@ QM_DEV void resolve_tops(PathScratch& w, Emitter& e, int n, float x_limit, bool 
         if (!(c.flags & 4u)) {
             if (!all && !(c.x < x_limit)) break;
             Cand k;
-            if (!profile_query(w.profile, c.x, k)) failed |= QM_FAILED_CAPACITY;
+            if (!profile_query(w.profile, c.x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
             c.z_at = k.z_at;
             c.g_at = k.g_at;
             c.pz = k.pz;
@@ -97,7 +97,7 @@ QM_DEV void crest_vertex(PathScratch& w, CrestGroup& cg, Emitter& e, int n, P2f 
     }
     if (!middle) {
         Cand k;
-        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY;
+        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
         k.x = x;
         k.z = z;
         tops_before(w, e, n, S, R, length, x, failed);
@@ -106,7 +106,7 @@ QM_DEV void crest_vertex(PathScratch& w, CrestGroup& cg, Emitter& e, int n, P2f 
     }
     if (!cg.open || z > cg.best.z) {
         Cand k;
-        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY;
+        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
         k.x = x;
         k.z = z;
         cg.best = k;
@@ -255,7 +255,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
                 continue;
             }
             if (nroofs >= ROOF_CAP) {
-                failed |= QM_FAILED_CAPACITY;
+                failed |= QM_FAILED_CAPACITY | QM_DETAIL_ROOFS;
             } else {
                 const Crossing& e = w.cross[open];
                 Roof r;
@@ -339,7 +339,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     bool last = false;
     while (!last) {
         if (--budget < 0) {
-            failed |= QM_FAILED_CAPACITY;
+            failed |= QM_FAILED_CAPACITY | QM_DETAIL_WALK;
             break;
         }
         float t;

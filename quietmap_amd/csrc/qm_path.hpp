@@ -335,7 +335,7 @@ QM_DEV uint32_t find_crossings(const qm_obstacles& O, double fx, double fy, doub
                             const double u = (wx * delta[1] - wy * delta[0]) / den;
                             if (t > 0.0 && t < 1.0 && u >= 0.0 && u <= 1.0) {
                                 if (n >= CROSS_CAP) {
-                                    failed |= QM_FAILED_CAPACITY;
+                                    failed |= QM_FAILED_CAPACITY | QM_DETAIL_CROSSINGS;
                                 } else {
                                     const uint32_t edge = (uint32_t)(next - 1);
                                     int k = n;
@@ -644,7 +644,7 @@ QM_DEV void hull_push(StateStream& st, P2f S, const Cand& c, uint32_t& failed) {
         else break;
     }
     if (st.hull_n >= HULL_CAP) {
-        failed |= QM_FAILED_CAPACITY;
+        failed |= QM_FAILED_CAPACITY | QM_DETAIL_HULL;
         return;
     }
     st.hull[st.hull_n++] = c;

@@ -36,6 +36,12 @@
 #define QM_FAILED_NOT_READ 1u   /* the ray leaves the scene's obstacle cells */
 #define QM_FAILED_NO_TERRAIN 2u /* a terrain node outside the lattice */
 #define QM_FAILED_CAPACITY 4u   /* a per-path buffer (crossings, hull, roofs) too small */
+/* With QM_FAILED_CAPACITY, which buffer (diagnostic detail; callers may mask it off). */
+#define QM_DETAIL_CROSSINGS 16u
+#define QM_DETAIL_ROOFS 32u
+#define QM_DETAIL_HULL 64u
+#define QM_DETAIL_PROFILE 128u
+#define QM_DETAIL_WALK 256u
 
 #define QM_KIND_BUILDING 0u
 #define QM_KIND_BARRIER 1u

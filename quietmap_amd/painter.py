@@ -34,6 +34,7 @@ EXPORTS = (
     "qm_paint_receivers_device",
     "qm_loudness",
     "qm_loudness_device",
+    "qm_aircraft_grid",
     "qm_crossings",
 )
 RECEIVER_BYTES = 256
@@ -68,6 +69,7 @@ def library():
         lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
+        lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, cp, sz]
         _lib = lib
     return _lib
 
@@ -195,6 +197,26 @@ class Gpu:
         rc = library().qm_loudness_device(self.handle, d_levels, n, cell_stride, band_stride, d_sone,
                                           stream or None, err, 512)
         _check(rc, err)
+
+    def aircraft_grid(self, scene):
+        """Doc 29 aircraft outputs of an AircraftScene's receiver grid: SEL energy per period,
+        N65, N70 and the LAmax histogram per receiver."""
+        nx, ny = scene.grid_shape
+        xy = np.ascontiguousarray(scene.points_xy, np.float64).reshape(-1, 2)
+        assert xy.shape[0] == nx * ny
+        classes = np.ascontiguousarray(scene.classes, abi.AIRCRAFT_CLASS_DTYPE)
+        tracks = np.ascontiguousarray(scene.tracks, abi.TRACK_DTYPE)
+        verts = np.ascontiguousarray(scene.vertices, np.float32).reshape(-1, 3)
+        n = nx * ny
+        energy = np.zeros((n, 3))
+        n65, n70 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        hist = np.zeros((n, abi.HIST_BINS), np.uint32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_aircraft_grid(self.handle, _ptr(classes), classes.size, _ptr(tracks), tracks.size,
+                                        _ptr(verts), verts.shape[0], _ptr(xy), nx, ny, scene.receiver_altitude_m,
+                                        scene.reach_m, _ptr(energy), _ptr(n65), _ptr(n70), _ptr(hist), err, 512)
+        _check(rc, err)
+        return dict(period_energy=energy, n65=n65, n70=n70, hist=hist)
 
     def crossings(self, square: Square, segments, capacity: int = 48):
         seg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)

@@ -1,0 +1,70 @@
+"""One launch of each kernel on a fixed workload, for rocprofv3 (kernel traces and counters).
+
+    python -m quietmap_amd.profile_workload --kernels paint,loudness,aircraft,copy --scale 0.125
+
+`paint` paints the tile-scale scene at `--scale` (one paint_kernel launch), `loudness` runs
+zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
+`--aircraft-scale`, and `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
+FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
+line when it ends.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import hip, scenes
+from .painter import RECEIVER_BYTES, Gpu
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kernels", default="paint,loudness,aircraft,copy")
+    ap.add_argument("--scale", type=float, default=0.125)
+    ap.add_argument("--cells", type=int, default=16 * 1024 * 1024)
+    ap.add_argument("--aircraft-scale", type=float, default=0.05)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    args = ap.parse_args()
+    kernels = args.kernels.split(",")
+    gpu = Gpu(0)
+    if "paint" in kernels:
+        sc = scenes.tile_scene(scale=args.scale)
+        nx, ny = sc.grid_shape
+        n = nx * ny
+        square = gpu.upload(sc.scene, sc.sources)
+        d_points = hip.DeviceBuffer.from_array(sc.points)
+        d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
+        d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+        gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)
+        t = time.time()
+        gpu.paint_receivers_device(square, d_rec.ptr, nx, ny, sc.cutoff_m, d_e.ptr, d_p.ptr, d_f.ptr)
+        hip.synchronize()
+        paths = int(d_p.to_array(np.uint32, n).astype(np.int64).sum())
+        print(f"paint: {nx}x{ny} receivers, {paths} paths, {time.time() - t:.2f} s", flush=True)
+    if "loudness" in kernels:
+        planes = np.random.default_rng(4).uniform(20.0, 90.0, (28, args.cells)).astype(np.float32)
+        d_levels = hip.DeviceBuffer.from_array(planes)
+        d_sone = hip.DeviceBuffer(args.cells * 4)
+        gpu.loudness_device(d_levels.ptr, args.cells, 1, args.cells, d_sone.ptr)
+        hip.synchronize()
+        print(f"loudness: {args.cells} cells, {args.cells * 116} algorithmic bytes", flush=True)
+    if "aircraft" in kernels:
+        ac = scenes.aircraft_scene(scale=args.aircraft_scale)
+        t = time.time()
+        r = gpu.aircraft_grid(ac)
+        print(f"aircraft: {ac.grid_shape} receivers, {ac.tracks.size} tracks, events binned "
+              f"{int(r['hist'].sum())}, {time.time() - t:.2f} s", flush=True)
+    if "copy" in kernels:
+        nbytes = args.copy_mib << 20
+        a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+        hip.check(hip.runtime().hipMemcpy(b.ptr, a.ptr, nbytes, hip.D2D))
+        hip.synchronize()
+        print(f"copy: {nbytes} bytes device to device", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -286,3 +286,74 @@ def tile_scene(seed=2, scale=1.0, receivers=4096, spacing=10.0, sources=200_000,
     _own_footprints(points, rings, footprints, spacing, (spacing / 2, spacing / 2), (nrec, nrec))
     scene = abi.Scene(terrain, obstacles, weather)
     return SyntheticScene("tile", scene, src, points, (nrec, nrec), cutoff_m, rings)
+
+
+class AircraftScene:
+    """Receivers, noise classes, tracks and their vertices of the aircraft configuration."""
+
+    def __init__(self, points_xy, grid_shape, classes, tracks, vertices, receiver_altitude_m, reach_m):
+        self.points_xy = points_xy
+        self.grid_shape = grid_shape
+        self.classes, self.tracks, self.vertices = classes, tracks, vertices
+        self.receiver_altitude_m, self.reach_m = receiver_altitude_m, reach_m
+
+
+def aircraft_scene(seed=3, scale=1.0, receivers=2048, spacing=50.0, classes=100, tracks=20_000, segments=200,
+                   segment_m=250.0, reach_m=16_000.0):
+    """Configuration "Doc 29 aircraft": 100 noise classes with NPD tables (10 distances x 4 power
+    settings per operation, SEL at 1,000 ft ~ N(90, 8) dB), 20,000 tracks of 200 segments with 3 degree
+    climbs and descents from four airports, a 2048x2048 receiver grid at 50 m (4 m above the datum),
+    16 km reach. ``scale`` shrinks the grid side and the track count with the area."""
+    rng = np.random.default_rng(seed)
+    n = max(int(round(receivers * scale)), 4)
+    extent = n * spacing
+    n_tracks = max(int(round(tracks * (n / receivers) ** 2)), 8)
+    d_ft = np.array([200.0, 400.0, 630.0, 1000.0, 2000.0, 4000.0, 6300.0, 10000.0, 16000.0, 25000.0])
+    cl = np.zeros(classes, abi.AIRCRAFT_CLASS_DTYPE)
+    for k in range(classes):
+        u = rng.uniform()
+        cl[k]["installation"] = abi.INSTALL_WING if u < 0.5 else (abi.INSTALL_FUSELAGE if u < 0.8 else abi.INSTALL_PROPELLER)
+        cl[k]["v_ref_kt"] = rng.uniform(140.0, 180.0)
+        placeholder = rng.uniform() < 0.05
+        for dep in range(2):
+            cl[k]["rows"][dep] = abi.POWER_ROWS
+            base = np.clip(rng.normal(90.0, 8.0), 65.0, 115.0) + (3.0 if dep else 0.0)
+            slope = rng.uniform(13.0, 19.0)
+            absorb = rng.uniform(0.0006, 0.0012)
+            for r in range(abi.POWER_ROWS):
+                sel = base + 2.0 * r - slope * np.log10(d_ft / 1000.0) - absorb * (d_ft - 1000.0)
+                delta = 12.0 if placeholder else 8.0 + 6.0 * np.log10(d_ft / 1000.0) + rng.uniform(-1.0, 1.0)
+                cl[k]["sel_db"][dep, r] = sel
+                cl[k]["lamax_db"][dep, r] = sel - delta
+    airports = rng.uniform(0.2 * extent, 0.8 * extent, (4, 2))
+    headings = rng.uniform(0.0, np.pi, 4)
+    tr = np.zeros(n_tracks, abi.TRACK_DTYPE)
+    verts = np.zeros((n_tracks, segments + 1, 3), np.float32)
+    tan3 = np.tan(np.radians(3.0))
+    for t in range(n_tracks):
+        a = rng.integers(0, 4)
+        dep = int(rng.uniform() < 0.5)
+        heading = headings[a] + (np.pi if rng.uniform() < 0.5 else 0.0)
+        turn = rng.normal(0.0, 0.004, segments)
+        turn[: segments // 10] = 0.0
+        head = heading + np.cumsum(turn)
+        steps = np.stack([np.cos(head), np.sin(head)], 1) * segment_m
+        path = np.concatenate([[airports[a]], airports[a] + np.cumsum(steps, 0)])
+        along = np.arange(segments + 1) * segment_m
+        z = np.minimum(along * tan3, 3000.0)
+        if not dep:  # an approach: flown toward the threshold, descending
+            path, z = path[::-1], z[::-1]
+        verts[t, :, 0], verts[t, :, 1], verts[t, :, 2] = path[:, 0], path[:, 1], z
+        tr[t]["first_vertex"] = t * (segments + 1)
+        tr[t]["vertex_count"] = segments + 1
+        tr[t]["aircraft_class"] = rng.integers(0, classes)
+        tr[t]["departure"] = dep
+        if dep:
+            tr[t]["power_row"], tr[t]["power_weight"] = 2, rng.uniform(0.0, 1.0)
+        else:
+            tr[t]["power_row"], tr[t]["power_weight"] = rng.integers(0, 2), rng.uniform(0.0, 1.0)
+        tr[t]["speed_kt"] = rng.uniform(140.0, 220.0)
+        tr[t]["flights"] = [rng.poisson(3.0), rng.poisson(1.0), rng.poisson(0.5)]
+    jj, ii = np.mgrid[0:n, 0:n]
+    xy = np.stack([(ii.reshape(-1) + 0.5) * spacing, (jj.reshape(-1) + 0.5) * spacing], 1).astype(np.float64)
+    return AircraftScene(np.ascontiguousarray(xy), (n, n), cl, tr, verts.reshape(-1, 3), 4.0, reach_m)

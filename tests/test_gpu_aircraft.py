@@ -1,0 +1,44 @@
+"""The HIP Doc 29 aircraft grid against the FP64 oracle on the same seeded scene: Lden within
+0.05 dB, N65/N70 event counts and the LAmax histogram bins bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from quietmap_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from quietmap_amd.painter import Gpu
+
+    g = Gpu(0)
+    yield g
+    g.close()
+
+
+@pytest.mark.parametrize("scale", [0.02, 0.05])
+def test_aircraft_grid_matches_oracle(gpu, scale):
+    print(f"case: aircraft scale {scale}", flush=True)
+    sc = scenes.aircraft_scene(scale=scale)
+    got = gpu.aircraft_grid(sc)
+    rng = np.random.default_rng(1)
+    n = sc.points_xy.shape[0]
+    sample = rng.choice(n, min(400, n), replace=False)
+    want = oracle.aircraft_points(sc, sc.points_xy[sample])
+    assert np.array_equal(got["n65"][sample], want["n65"])
+    assert np.array_equal(got["n70"][sample], want["n70"])
+    assert np.array_equal(got["hist"][sample], want["hist"])
+    from quietmap_amd.painter import lden_db
+
+    hours = np.array([12.0, 4.0, 8.0]) * 3600.0
+    g = lden_db(got["period_energy"][sample] / hours)
+    w = lden_db(want["period_energy"] / hours)
+    ok = np.isfinite(w)
+    assert np.array_equal(np.isfinite(g), ok)
+    err = np.abs(g[ok] - w[ok])
+    print(f"aircraft Lden: n={ok.sum()} max={err.max():.5f} dB; events binned {int(want['hist'].sum())}, "
+          f"N65 total {int(want['n65'].sum())}", flush=True)
+    assert err.max() <= 0.05
+    assert want["hist"].sum() > 100
