@@ -53,10 +53,7 @@ def main():
     t0 = time.time()
     sc = scenes.tile_scene(scale=args.scale)
     gx, gy = sc.grid_shape
-    nx, ny = min(args.window, gx), min(args.window, gy)
-    x0, y0 = (gx - nx) // 2, (gy - ny) // 2
-    window = ((np.arange(ny)[:, None] + y0) * gx + (np.arange(nx)[None, :] + x0)).reshape(-1)
-    points = np.ascontiguousarray(sc.points[window])
+    points, nx, ny = sc.window(args.window)
     n = nx * ny
     print(f"[bench] scene {nx}x{ny} receivers, {sc.sources.size} sources, "
           f"{sc.scene.obstacles.outlines.size} buildings, generated in {time.time() - t0:.1f} s", file=sys.stderr,

@@ -46,6 +46,12 @@ struct BandTable {
 };
 
 QM_DEV float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// The hardware's 1-ulp square root and reciprocal (v_sqrt_f32, v_rcp_f32) for the FP32 geometry of
+// the walk and the candidates, where the IEEE-rounded forms cost a correction sequence each.
+QM_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+QM_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+QM_DEV float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 QM_DEV double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
 
 // physics/src/bands.rs:36-43 energy, amplitude

@@ -11,7 +11,7 @@ namespace qm {
 struct PathScratch {
     Crossing cross[CROSS_CAP];
     Roof roofs[ROOF_CAP];
-    Profile profile;
+    CandLog log;
     StateStream state[2];
 };
 
@@ -22,9 +22,11 @@ struct CrestGroup {
     Cand best;
 };
 
-struct Emitter {
-    int next_top;  // next crossing whose top is not yet a candidate
-    int tops_resolved;
+// Wall tops in distance order: the next to read the profile at (`resolved`) and the next to
+// emit as a candidate (`emitted`), with their x (+inf when none is left).
+struct Tops {
+    int resolved, emitted;
+    float resolve_x, emit_x;
 };
 
 // Favourable probability of `period` for sound travelling along `azimuth` (weather.rs:61-70).
@@ -39,22 +41,25 @@ QM_DEV float favourable_at(const float* row, float azimuth) {
     return row[lower] + fraction * (row[upper] - row[lower]);
 }
 
-QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, const Cand& c, uint32_t& failed) {
+QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
     if (!(c.x > 0.0f && c.x < length)) return;
-    state_candidate(w.state[0], S, R, c, failed);
-    state_candidate(w.state[1], S, R, c, failed);
+    c.id = -1;
+    state_candidate(w.state, 0, w.log, S, R, c, failed);
+    state_candidate(w.state, 1, w.log, S, R, c, failed);
+}
+
+// The x of the first top at or after index i not dropped (+inf when none).
+QM_DEV float top_x_from(const PathScratch& w, int& i, int n) {
+    while (i < n && (w.cross[i].flags & 4u)) ++i;
+    return i < n ? w.cross[i].x : __builtin_inff();
 }
 
 // Emits every resolved top before x (the candidate merge of physics/src/cnossos/mod.rs:160-184:
 // a top first only when strictly nearer).
-QM_DEV void tops_before(PathScratch& w, Emitter& e, int n, P2f S, P2f R, float length, float x, uint32_t& failed) {
-    while (e.next_top < n) {
-        const Crossing& c = w.cross[e.next_top];
-        if (c.flags & 4u) {
-            ++e.next_top;
-            continue;
-        }
-        if (!(c.x < x) || e.next_top >= e.tops_resolved) break;
+__device__ __noinline__ void tops_before_slow(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x,
+                                              uint32_t& failed) {
+    while (tp.emit_x < x && tp.emitted < tp.resolved) {
+        const Crossing& c = w.cross[tp.emitted];
         Cand k;
         k.x = c.x;
         k.z = c.top;
@@ -64,52 +69,60 @@ QM_DEV void tops_before(PathScratch& w, Emitter& e, int n, P2f S, P2f R, float l
         k.pxz = c.pxz;
         k.pg = c.pg;
         candidate_out(w, S, R, length, k, failed);
-        ++e.next_top;
+        ++tp.emitted;
+        tp.emit_x = top_x_from(w, tp.emitted, n);
+    }
+}
+QM_DEV void tops_before(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x, uint32_t& failed) {
+    if (tp.emit_x < x && tp.emitted < tp.resolved) tops_before_slow(w, tp, n, S, R, length, x, failed);
+}
+
+// Reads the profile for every top before x on the segment from the profile's head to the vertex
+// (x, z, g) about to be emitted.
+__device__ __noinline__ void resolve_tops_slow(PathScratch& w, Tops& tp, int n, const Profile& P, float x, float z,
+                                               float g) {
+    while (tp.resolve_x < x) {
+        Crossing& c = w.cross[tp.resolved];
+        Cand k;
+        segment_at(P.head, x, z, g, c.x, P.z0, k);
+        c.z_at = k.z_at;
+        c.g_at = k.g_at;
+        c.pz = k.pz;
+        c.pxz = k.pxz;
+        c.pg = k.pg;
+        ++tp.resolved;
+        tp.resolve_x = top_x_from(w, tp.resolved, n);
     }
 }
 
-// Resolves (profile and prefix at x) every top before `x_limit` not yet resolved.
-QM_DEV void resolve_tops(PathScratch& w, Emitter& e, int n, float x_limit, bool all, uint32_t& failed) {
-    while (e.tops_resolved < n) {
-        Crossing& c = w.cross[e.tops_resolved];
-        if (!(c.flags & 4u)) {
-            if (!all && !(c.x < x_limit)) break;
-            Cand k;
-            if (!profile_query(w.profile, c.x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
-            c.z_at = k.z_at;
-            c.g_at = k.g_at;
-            c.pz = k.pz;
-            c.pxz = k.pxz;
-            c.pg = k.pg;
-        }
-        ++e.tops_resolved;
-    }
+// Appends a vertex to the merged profile (tops before it read their profile first).
+QM_DEV void profile_emit(PathScratch& w, Tops& tp, int n, Profile& P, float x, float z, float g) {
+    if (tp.resolve_x < x) resolve_tops_slow(w, tp, n, P, x, z, g);
+    P.head = extend(P.head, x, z, g, P.z0);
 }
 
-QM_DEV void crest_vertex(PathScratch& w, CrestGroup& cg, Emitter& e, int n, P2f S, P2f R, float length, float x,
-                         float z, uint32_t& failed) {
+QM_DEV void crest_vertex(PathScratch& w, const Profile& P, CrestGroup& cg, Tops& tp, int n, P2f S, P2f R,
+                         float length, float x, float z, uint32_t& failed) {
     const bool middle = x > CREST_ENDS_M && x < length - CREST_ENDS_M;
     const int group = (int)floorf(x / CREST_SPACING_M);
     if (cg.open && (!middle || cg.group != group)) {
-        tops_before(w, e, n, S, R, length, cg.best.x, failed);
+        tops_before(w, tp, n, S, R, length, cg.best.x, failed);
         candidate_out(w, S, R, length, cg.best, failed);
         cg.open = false;
     }
     if (!middle) {
         Cand k;
-        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
+        profile_at_walk(P, x, k);
         k.x = x;
         k.z = z;
-        tops_before(w, e, n, S, R, length, x, failed);
+        tops_before(w, tp, n, S, R, length, x, failed);
         candidate_out(w, S, R, length, k, failed);
         return;
     }
     if (!cg.open || z > cg.best.z) {
-        Cand k;
-        if (!profile_query(w.profile, x, k)) failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
-        k.x = x;
-        k.z = z;
-        cg.best = k;
+        profile_at_walk(P, x, cg.best);
+        cg.best.x = x;
+        cg.best.z = z;
         cg.group = group;
         cg.open = true;
     }
@@ -118,30 +131,32 @@ QM_DEV void crest_vertex(PathScratch& w, CrestGroup& cg, Emitter& e, int n, P2f 
 // A ray's foliage depth in one state (foliage.rs:36-64), streamed over the terrain samples.
 struct Foliage {
     float depth[2];
-    float prev_x, prev_z[2], prev_w[2], prev_cover;
+    float prev_x, prev_cover, prev_ground, prev_canopy;
 };
 
-QM_DEV void foliage_sample(Foliage& f, const PathScratch& w, P2f S, P2f R, float length, float x, float ground,
-                           float cover, float canopy, bool first) {
-    float zs[2], ws[2];
-    for (int s = 0; s < 2; ++s) {
-        const float radius = w.state[s].radius;
-        zs[s] = S.z + (x / length) * (R.z - S.z) + sag_f(radius, S, R, x);
-        const float above = zs[s] - ground;
-        ws[s] = (above > 0.0f && above <= canopy) ? cover : 0.0f;
+// Points are taken lazily: a stretch counts only where either end has cover, so a ray over open
+// ground computes nothing here.
+QM_DEV void foliage_sample(Foliage& f, const PathScratch& w, float x, float ground, float cover, float canopy,
+                           bool first) {
+    if (first || !(f.prev_cover > 0.0f || cover > 0.0f)) {
+        f.prev_x = x;
+        f.prev_cover = cover;
+        f.prev_ground = ground;
+        f.prev_canopy = canopy;
+        return;
     }
-    if (!first && (f.prev_cover > 0.0f || cover > 0.0f)) {
-        for (int s = 0; s < 2; ++s) {
-            const float dx = x - f.prev_x, dz = zs[s] - f.prev_z[s];
-            f.depth[s] += 0.5f * (f.prev_w[s] + ws[s]) * sqrtf(dx * dx + dz * dz);
-        }
+    for (int s = 0; s < 2; ++s) {
+        const float za = line_altitude(w.state[s].line, f.prev_x), zb = line_altitude(w.state[s].line, x);
+        const float above_a = za - f.prev_ground, above_b = zb - ground;
+        const float wa = (above_a > 0.0f && above_a <= f.prev_canopy) ? f.prev_cover : 0.0f;
+        const float wb = (above_b > 0.0f && above_b <= canopy) ? cover : 0.0f;
+        const float dx = x - f.prev_x, dz = zb - za;
+        f.depth[s] += 0.5f * (wa + wb) * fsqrt(dx * dx + dz * dz);
     }
     f.prev_x = x;
     f.prev_cover = cover;
-    for (int s = 0; s < 2; ++s) {
-        f.prev_z[s] = zs[s];
-        f.prev_w[s] = ws[s];
-    }
+    f.prev_ground = ground;
+    f.prev_canopy = canopy;
 }
 
 // terrain_at and ground_factor_at of physics/src/ray.rs:265-283 at x = t * length, from the two
@@ -309,6 +324,8 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         const float direct = sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt));
         w.state[0].radius = 0.0f;
         w.state[1].radius = fmaxf(1000.0f, 8.0f * direct);
+        w.state[0].line = state_line(w.state[0].radius, S, R);
+        w.state[1].line = state_line(w.state[1].radius, S, R);
         for (int s = 0; s < 2; ++s) {
             w.state[s].blocking = false;
             w.state[s].has_pending = false;
@@ -317,20 +334,29 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             w.state[s].best_delta = 0.0f;
         }
     }
-    w.profile.count = 0;
-    Emitter e{0, 0};
+    w.log.n = 0;
+    Tops tp;
+    tp.resolved = tp.emitted = 0;
+    tp.resolve_x = top_x_from(w, tp.resolved, n);
+    tp.emitted = tp.resolved;
+    tp.emit_x = tp.resolve_x;
     CrestGroup cg;
     cg.open = false;
     cg.group = 0;
     Foliage fol;
     fol.depth[0] = fol.depth[1] = 0.0f;
     int r = 0;
+    float next_roof = nroofs > 0 ? w.roofs[0].x0 : __builtin_inff();
     float covered_to = 0.0f;
 
     // Vertex 0: the source's ground.
-    profile_emit(w.profile, 0.0f, s0.z, s0.g);
-    foliage_sample(fol, w, S, R, H, 0.0f, s0.z, s0.forest, s0.canopy, true);
-    crest_vertex(w, cg, e, n, S, R, H, 0.0f, s0.z, failed);
+    Profile P;
+    P.z0 = s0.z;
+    P.head = Vertex{0.0f, s0.z, s0.g, 0.0, 0.0, 0.0};
+    P.roof_near = P.head;
+    P.roof_x1 = P.roof_top1 = 0.0f;
+    foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
+    crest_vertex(w, P, cg, tp, n, S, R, H, 0.0f, s0.z, failed);
 
     int next_column = L.first_column, next_row = L.first_row, south = L.start_south;
     const float inf = __builtin_inff();
@@ -368,43 +394,61 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         }
         const float x = t * H;
         const float z = x < platform ? fminf(v.z, source_ground) : v.z;
-        foliage_sample(fol, w, S, R, H, x, v.z, v.forest, v.canopy, false);
+        foliage_sample(fol, w, x, v.z, v.forest, v.canopy, false);
         // Roofs starting before this vertex, then the vertex unless a roof covers it.
-        while (r < nroofs && w.roofs[r].x0 < x) {
-            const Roof& f = w.roofs[r];
-            profile_emit(w.profile, f.x0, f.ground0, f.g0);
-            profile_emit(w.profile, f.x0, f.top0, 0.0f);
-            profile_emit(w.profile, f.x1, f.top1, 0.0f);
-            profile_emit(w.profile, f.x1, f.ground1, f.g1);
+        while (next_roof < x) {
+            const Roof f = w.roofs[r];
+            profile_emit(w, tp, n, P, f.x0, f.ground0, f.g0);
+            profile_emit(w, tp, n, P, f.x0, f.top0, 0.0f);
+            P.roof_near = P.head;
+            profile_emit(w, tp, n, P, f.x1, f.top1, 0.0f);
+            profile_emit(w, tp, n, P, f.x1, f.ground1, f.g1);
+            P.roof_x1 = f.x1;
+            P.roof_top1 = f.top1;
             covered_to = f.x1;
             ++r;
+            next_roof = r < nroofs ? w.roofs[r].x0 : inf;
         }
-        if (x > covered_to) profile_emit(w.profile, x, z, v.g);
-        resolve_tops(w, e, n, x, false, failed);
-        crest_vertex(w, cg, e, n, S, R, H, x, z, failed);
+        if (x > covered_to) profile_emit(w, tp, n, P, x, z, v.g);
+        crest_vertex(w, P, cg, tp, n, S, R, H, x, z, failed);
     }
     if (failed & (QM_FAILED_NO_TERRAIN | QM_FAILED_CAPACITY)) {
         out.failed = failed;
         return;
     }
     if (cg.open) {
-        tops_before(w, e, n, S, R, H, cg.best.x, failed);
+        tops_before(w, tp, n, S, R, H, cg.best.x, failed);
         candidate_out(w, S, R, H, cg.best, failed);
     }
-    resolve_tops(w, e, n, 0.0f, true, failed);
-    tops_before(w, e, n, S, R, H, inf, failed);
+    // Tops at the last vertex's x take it; none can lie beyond it.
+    while (tp.resolved < n) {
+        Crossing& c = w.cross[tp.resolved];
+        if (!(c.flags & 4u)) {
+            if (c.x != P.head.x) {
+                failed |= QM_FAILED_CAPACITY | QM_DETAIL_PROFILE;
+                break;
+            }
+            c.z_at = P.head.z;
+            c.g_at = P.head.g;
+            c.pz = P.head.pz;
+            c.pxz = P.head.pxz;
+            c.pg = P.head.pg;
+        }
+        ++tp.resolved;
+    }
+    tops_before(w, tp, n, S, R, H, inf, failed);
 
     // The direct path (diffraction.rs:46-63) from the whole profile's prefix.
-    const RingVertex& end = w.profile.ring[(w.profile.count - 1) % RING];
+    const Vertex& end = P.head;
     const double Ld = (double)H;
-    const double z0 = w.profile.z0;
+    const double z0 = P.z0;
     const Plane direct_plane = plane_over(0.0, Ld, z0, z0, 0.0, 0.0, end.pz, end.pxz);
     const Geometry direct = geometry_over(direct_plane, P2d{0.0, (double)source_alt}, P2d{Ld, (double)receiver_alt});
     const float path_ground = (float)(end.pg / Ld);
     const float near_source = near_source_factor(direct, path_ground, source_g);
     float att[2][BANDS];
     for (int s = 0; s < 2; ++s)
-        state_boundary(B, w.state[s], s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
+        state_boundary(B, w.state[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
                        near_source, source_g, s0.g, failed, att[s]);
 
     // ray.rs:166-190: foliage, air at the slant distance, the period mix; evaluate.rs:119-152.

@@ -30,7 +30,7 @@ namespace qm {
 constexpr int CROSS_CAP = 48;
 constexpr int ROOF_CAP = 24;
 constexpr int HULL_CAP = 24;
-constexpr int RING = 16;
+constexpr int LOG_CAP = 64;  // candidate log entries; compacted when full (live entries <= 2*HULL_CAP + 4)
 
 struct Scene {
     qm_terrain terrain;
@@ -45,12 +45,15 @@ struct GroundSample {
 // ---------------------------------------------------------------------------------------------
 // Terrain: tiles/src/terrain.rs:36-38 height code, popup/src/scene.rs:532-561 edge_sample,
 // tiles/src/terrain.rs:342-385 sample_at (fractions north and east of the south-west node).
-QM_DEV bool node_at(const qm_terrain& T, int row, int column, uint32_t& node, uint32_t& canopy) {
+QM_DEV bool node_at(const qm_terrain& T, int row, int column, uint32_t& node, uint32_t& index) {
     if (row < 0 || column < 0 || row >= (int)T.rows || column >= (int)T.columns) return false;
-    const size_t index = (size_t)row * T.columns + (size_t)column;
-    node = __builtin_nontemporal_load(T.nodes + index);
-    canopy = T.canopy_m[index];
+    index = (uint32_t)row * T.columns + (uint32_t)column;
+    node = T.nodes[index];
     return true;
+}
+// The canopy byte is read only under forest (the only reader is the foliage term).
+QM_DEV float canopy_of(const qm_terrain& T, uint32_t node, uint32_t index) {
+    return (node >> 24) ? (float)T.canopy_m[index] : 0.0f;
 }
 QM_DEV bool node_valid(uint32_t n) { return (n & 0xffffu) != 0xffffu && ((n >> 16) & 0xffu) <= 100u; }
 QM_DEV float height_of(uint32_t n) { return -500.0f + (float)(n & 0xffffu) * 0.2f; }
@@ -72,14 +75,14 @@ QM_DEV uint32_t edge_sample(const qm_terrain& T, int r0, int c0, int r1, int c1,
         }
     }
     const uint32_t nearer = f < 0.5f ? a : b;
-    const uint32_t nearer_canopy = f < 0.5f ? ca : cb;
+    const uint32_t nearer_index = f < 0.5f ? ca : cb;
     if ((nearer >> 24) > 100u || !node_valid(a) || !node_valid(b)) return QM_FAILED_NO_TERRAIN;
     const float za = height_of(a), zb = height_of(b);
     const float ia = impervious_of(a), ib = impervious_of(b);
     out.z = za + f * (zb - za);
     out.g = 1.0f - (ia + f * (ib - ia)) * 0.01f;
     out.forest = forest_of(nearer) * 0.01f;
-    out.canopy = (float)nearer_canopy;
+    out.canopy = canopy_of(T, nearer, nearer_index);
     return 0;
 }
 
@@ -106,14 +109,14 @@ QM_DEV uint32_t ground_at(const qm_terrain& T, double cp, double rp, GroundSampl
     out.z = h;
     out.g = 1.0f - imp * 0.01f;
     out.forest = forest_of(n[nearest]) * 0.01f;
-    out.canopy = (float)k[nearest];
+    out.canopy = canopy_of(T, n[nearest], k[nearest]);
     return 0;
 }
 
 // The ray's lattice geometry: position of the source (ca, ra) in lattice units, the offsets to
 // the receiver, and the first lines strictly ahead of the source.
 struct Lattice {
-    float ca, ra, columns, rise;
+    float ca, ra, columns, rise, inv_columns, inv_rise;
     int first_column, column_step, first_row, row_step, start_south;
 };
 
@@ -126,6 +129,8 @@ QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, 
     L.ra = (float)ra;
     L.columns = (float)(cb - ca);
     L.rise = (float)(rb - ra);
+    L.inv_columns = L.columns != 0.0f ? 1.0f / L.columns : 0.0f;
+    L.inv_rise = L.rise != 0.0f ? 1.0f / L.rise : 0.0f;
     if (L.columns > 0.0f) {
         L.first_column = (int)floorf(L.ca) + 1;
         L.column_step = 1;
@@ -149,8 +154,8 @@ QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, 
     return L;
 }
 
-QM_DEV float t_of_column(const Lattice& L, int c) { return ((float)c - L.ca) / L.columns; }
-QM_DEV float t_of_row(const Lattice& L, int k) { return ((float)k - L.ra) / L.rise; }
+QM_DEV float t_of_column(const Lattice& L, int c) { return ((float)c - L.ca) * L.inv_columns; }
+QM_DEV float t_of_row(const Lattice& L, int k) { return ((float)k - L.ra) * L.inv_rise; }
 
 // Sample on row line k at fraction t.
 QM_DEV uint32_t row_vertex(const qm_terrain& T, const Lattice& L, int k, float t, GroundSample& out) {
@@ -403,22 +408,36 @@ struct P2d {
 };
 
 // A diffraction candidate: its point, the profile's altitude and ground factor just after it, and
-// the prefix moments of the profile from the source up to it (about (0, z0)).
+// the prefix moments of the profile from the source up to it (about (0, z0)). `id` is its entry
+// in the path's candidate log once a state has kept it, -1 before.
 struct Cand {
     float x, z, z_at, g_at;
     double pz, pxz, pg;
+    int id;
+};
+
+// What the boundary terms read of a kept candidate beyond its point (one log entry).
+struct CandData {
+    float z_at, g_at;
+    double pz, pxz, pg;
+};
+
+// A kept candidate as the states hold it: its point and its log entry.
+struct CandRef {
+    float x, z;
+    int id;
 };
 
 // physics/src/cnossos/rubber_band.rs:182-193, FP32 and FP64 forms.
 QM_DEV float triangle_excess_f(P2f s, P2f o, P2f r) {
     const float ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
-    const float lu = sqrtf(ux * ux + uz * uz), lv = sqrtf(vx * vx + vz * vz);
+    const float lu = fsqrt(ux * ux + uz * uz), lv = fsqrt(vx * vx + vz * vz);
     const float wx = ux + vx, wz = uz + vz;
-    const float lw = sqrtf(wx * wx + wz * wz);
+    const float lw = fsqrt(wx * wx + wz * wz);
     const float dot = ux * vx + uz * vz;
     if (dot <= 0.0f) return lu + lv - lw;
     const float cross = ux * vz - uz * vx;
-    return 2.0f * cross * cross / ((lu * lv + dot) * (lu + lv + lw));
+    return fdiv(2.0f * cross * cross, (lu * lv + dot) * (lu + lv + lw));
 }
 QM_DEV double triangle_excess_d(P2d s, P2d o, P2d r) {
     const double ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
@@ -432,7 +451,7 @@ QM_DEV double triangle_excess_d(P2d s, P2d o, P2d r) {
 }
 QM_DEV float dist_f(P2f a, P2f b) {
     const float dx = b.x - a.x, dz = b.z - a.z;
-    return sqrtf(dx * dx + dz * dz);
+    return fsqrt(dx * dx + dz * dz);
 }
 QM_DEV double dist_d(P2d a, P2d b) {
     const double dx = b.x - a.x, dz = b.z - a.z;
@@ -442,7 +461,7 @@ QM_DEV double dist_d(P2d a, P2d b) {
 // rubber_band.rs:65-80 arc_excess (favourable state only; radius > 0 marks favourable).
 QM_DEV float arc_excess_f(float radius, float chord) {
     if (radius <= 0.0f) return 0.0f;
-    const float x = chord / (2.0f * radius);
+    const float x = chord * frcp(2.0f * radius);
     if (x > 1.0f / 16.0f) return 2.0f * radius * asinf(fminf(x, 1.0f)) - chord;
     const float x2 = x * x;
     return 2.0f * radius * x * x2 *
@@ -462,7 +481,7 @@ QM_DEV float single_path_difference_f(float radius, P2f from, P2f o, P2f to) {
     const float arcs = arc_excess_f(radius, dist_f(from, o)) + arc_excess_f(radius, dist_f(o, to)) -
                        arc_excess_f(radius, dist_f(from, to));
     const float excess = chords + arcs;
-    const float chord = from.z + (to.z - from.z) * (o.x - from.x) / (to.x - from.x);
+    const float chord = from.z + fdiv((to.z - from.z) * (o.x - from.x), to.x - from.x);
     if (o.z >= chord) return excess;
     if (radius <= 0.0f) return -excess;
     const P2f a{o.x, chord};
@@ -503,12 +522,46 @@ QM_DEV float sag_f(float radius, P2f from, P2f to, float dx) {
     const float horizontal = to.x - from.x;
     const float chord = dist_f(from, to);
     const float gamma2 = radius * radius;
-    const float along_per_x = chord / horizontal;
-    const float centre_term = sqrtf(gamma2 - 0.25f * chord * chord);
+    const float along_per_x = fdiv(chord, horizontal);
+    const float centre_term = fsqrt(gamma2 - 0.25f * chord * chord);
     const float along = dx * along_per_x;
     const float offset = along - 0.5f * chord;
-    const float perpendicular = along * (chord - along) / (sqrtf(fmaxf(gamma2 - offset * offset, 0.0f)) + centre_term);
+    const float perpendicular = fdiv(along * (chord - along), fsqrt(fmaxf(gamma2 - offset * offset, 0.0f)) + centre_term);
     return perpendicular * along_per_x;
+}
+
+// A state's ray from S to R with its per-path constants taken once (rubber_band.rs:125-138): the
+// altitude of the straight chord, and the favourable arc's height above it.
+struct StateLine {
+    float sx, sz, slope;        // chord: sz + slope * (x - sx)
+    float along_per_x, chord, gamma2, centre_term;
+    bool favourable;
+};
+
+QM_DEV StateLine state_line(float radius, P2f from, P2f to) {
+    StateLine L;
+    L.sx = from.x;
+    L.sz = from.z;
+    L.slope = fdiv(to.z - from.z, to.x - from.x);
+    L.favourable = radius > 0.0f;
+    L.chord = dist_f(from, to);
+    L.gamma2 = radius * radius;
+    L.along_per_x = fdiv(L.chord, to.x - from.x);
+    L.centre_term = L.favourable ? fsqrt(L.gamma2 - 0.25f * L.chord * L.chord) : 0.0f;
+    return L;
+}
+
+// The state's ray altitude at horizontal distance x (Sag::at, rubber_band.rs:349-359).
+QM_DEV float line_altitude(const StateLine& L, float x) {
+    const float dx = x - L.sx;
+    float z = L.sz + L.slope * dx;
+    if (L.favourable) {
+        const float along = dx * L.along_per_x;
+        const float offset = along - 0.5f * L.chord;
+        z += fdiv(along * (L.chord - along), fsqrt(fmaxf(L.gamma2 - offset * offset, 0.0f)) + L.centre_term) *
+             L.along_per_x;
+    }
+    return z;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -623,23 +676,67 @@ QM_DEV float ground_split_db(float ground_db, float mirrored_db, float direct_db
 
 // The per-state streaming rubber band and unblocked-best search (rubber_band.rs:202-264).
 struct StateStream {
+    StateLine line;  // the state's ray from S to R
     float radius;  // 0 calm; the favourable arc's radius
     bool blocking;
     bool has_pending;
     bool have_best;
     int hull_n;    // entries in `hull` after the source
     float best_delta;
-    Cand pending;
-    Cand best;
-    Cand hull[HULL_CAP];
+    CandRef pending;
+    CandRef best;
+    CandRef hull[HULL_CAP];
 };
 
-QM_DEV void hull_push(StateStream& st, P2f S, const Cand& c, uint32_t& failed) {
+// The candidate log of one path: the data of every candidate a state has kept.
+struct CandLog {
+    CandData entry[LOG_CAP];
+    int n;
+};
+
+// Drops the entries no state refers to any more and renumbers the rest in order (the log is full;
+// at most 2*HULL_CAP + 4 entries are live, so this frees at least 12).
+__device__ __noinline__ void log_compact(CandLog& log, StateStream* states) {
+    uint64_t live = 0;
+    for (int s = 0; s < 2; ++s) {
+        const StateStream& st = states[s];
+        for (int i = 0; i < st.hull_n; ++i) live |= 1ull << st.hull[i].id;
+        if (st.has_pending) live |= 1ull << st.pending.id;
+        if (st.have_best && !st.blocking) live |= 1ull << st.best.id;
+    }
+    int next = 0;
+    for (int i = 0; i < LOG_CAP; ++i)
+        if ((live >> i) & 1ull) log.entry[next++] = log.entry[i];
+    log.n = next;
+    for (int s = 0; s < 2; ++s) {
+        StateStream& st = states[s];
+        for (int i = 0; i < st.hull_n; ++i) st.hull[i].id = __popcll(live & ((1ull << st.hull[i].id) - 1ull));
+        if (st.has_pending) st.pending.id = __popcll(live & ((1ull << st.pending.id) - 1ull));
+        if (st.have_best && !st.blocking) st.best.id = __popcll(live & ((1ull << st.best.id) - 1ull));
+    }
+}
+
+// The candidate as a state keeps it, logging its data on first use (shared by both states).
+QM_DEV CandRef keep(CandLog& log, StateStream* states, Cand& c) {
+    if (c.id < 0) {
+        if (log.n >= LOG_CAP) log_compact(log, states);
+        c.id = log.n++;
+        CandData& d = log.entry[c.id];
+        d.z_at = c.z_at;
+        d.g_at = c.g_at;
+        d.pz = c.pz;
+        d.pxz = c.pxz;
+        d.pg = c.pg;
+    }
+    return CandRef{c.x, c.z, c.id};
+}
+
+QM_DEV void hull_push(StateStream& st, P2f S, const CandRef& c, uint32_t& failed) {
     while (st.hull_n >= 1) {
         const P2f a = st.hull_n >= 2 ? P2f{st.hull[st.hull_n - 2].x, st.hull[st.hull_n - 2].z} : S;
         const P2f b{st.hull[st.hull_n - 1].x, st.hull[st.hull_n - 1].z};
         const P2f p{c.x, c.z};
-        const float ray_at_b = a.z + (p.z - a.z) * (b.x - a.x) / (p.x - a.x) + sag_f(st.radius, a, p, b.x - a.x);
+        const float ray_at_b = a.z + fdiv((p.z - a.z) * (b.x - a.x), p.x - a.x) + sag_f(st.radius, a, p, b.x - a.x);
         if (b.z <= ray_at_b) --st.hull_n;
         else break;
     }
@@ -650,16 +747,16 @@ QM_DEV void hull_push(StateStream& st, P2f S, const Cand& c, uint32_t& failed) {
     st.hull[st.hull_n++] = c;
 }
 
-// One candidate (in order of distance) into a state's stream.
-QM_DEV void state_candidate(StateStream& st, P2f S, P2f R, const Cand& c, uint32_t& failed) {
+// One candidate (in order of distance) into state `s`'s stream.
+QM_DEV void state_candidate(StateStream* states, int s, CandLog& log, P2f S, P2f R, Cand& c, uint32_t& failed) {
+    StateStream& st = states[s];
     const P2f p{c.x, c.z};
-    const float chord = S.z + (R.z - S.z) * (p.x - S.x) / (R.x - S.x);
-    if (p.z > chord + sag_f(st.radius, S, R, p.x - S.x)) {
+    if (p.z > line_altitude(st.line, p.x)) {
         if (st.has_pending && st.pending.x == p.x) {
-            if (p.z > st.pending.z) st.pending = c;
+            if (p.z > st.pending.z) st.pending = keep(log, states, c);
         } else {
             if (st.has_pending) hull_push(st, S, st.pending, failed);
-            st.pending = c;
+            st.pending = keep(log, states, c);
             st.has_pending = true;
         }
         st.blocking = true;
@@ -669,80 +766,68 @@ QM_DEV void state_candidate(StateStream& st, P2f S, P2f R, const Cand& c, uint32
     const float delta = single_path_difference_f(st.radius, S, p, R);
     if (!st.have_best || delta > st.best_delta) {
         st.best_delta = delta;
-        st.best = c;
+        st.best = keep(log, states, c);
         st.have_best = true;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// The merged profile: a ring of its last vertices with their prefix moments.
-struct RingVertex {
+// The merged profile as the walk streams it: the last vertex emitted with its prefix moments
+// (integrals from the source of z - z0, x*(z - z0) and G), and the top segment of the last roof
+// emitted, whose near end keeps its prefix, for the walk vertices that roof covers.
+struct Vertex {
     float x, z, g;
-    float pad;
     double pz, pxz, pg;
 };
 
 struct Profile {
-    RingVertex ring[RING];
-    int count;   // vertices emitted
-    double z0;   // altitude of the first vertex (the moments' origin)
+    Vertex head;       // the last vertex emitted
+    Vertex roof_near;  // the last roof's near-wall top
+    float roof_x1, roof_top1;
+    double z0;         // altitude of the first vertex (the moments' origin)
 };
 
-QM_DEV void profile_emit(Profile& P, float x, float z, float g) {
-    RingVertex v;
-    v.x = x;
-    v.z = z;
-    v.g = g;
-    v.pad = 0.0f;
-    if (P.count == 0) {
-        P.z0 = z;
-        v.pz = v.pxz = v.pg = 0.0;
-    } else {
-        const RingVertex& q = P.ring[(P.count - 1) % RING];
-        const double u0 = q.x, u1 = x;
-        const double width = u1 - u0;
-        v.pz = q.pz;
-        v.pxz = q.pxz;
-        v.pg = q.pg;
-        if (width > 0.0) {
-            const double w0 = (double)q.z - P.z0, w1 = (double)z - P.z0;
-            v.pz += 0.5 * (w0 + w1) * width;
-            v.pxz += width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
-            v.pg += 0.5 * ((double)q.g + (double)g) * width;
-        }
+// The vertex (x, z, g) after `a`, its prefix moments extended over the segment between them.
+QM_DEV Vertex extend(const Vertex& a, float x, float z, float g, double z0) {
+    Vertex v{x, z, g, a.pz, a.pxz, a.pg};
+    const double u0 = a.x, u1 = x;
+    const double width = u1 - u0;
+    if (width > 0.0) {
+        const double w0 = (double)a.z - z0, w1 = (double)z - z0;
+        v.pz += 0.5 * (w0 + w1) * width;
+        v.pxz += width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
+        v.pg += 0.5 * ((double)a.g + (double)g) * width;
     }
-    P.ring[P.count % RING] = v;
-    ++P.count;
+    return v;
 }
 
-// The profile at x (upper side of a step: the last vertex at or before x, interpolated toward the
-// next) and its prefix moments up to x, for x within the ring; false where x lies before the ring
-// or beyond the last vertex emitted.
-QM_DEV bool profile_query(const Profile& P, float x, Cand& c) {
-    const int oldest = P.count > RING ? P.count - RING : 0;
-    int j = P.count - 1;
-    while (j >= oldest && P.ring[j % RING].x > x) --j;
-    if (j < oldest) return false;
-    const RingVertex& a = P.ring[j % RING];
-    c.z_at = a.z;
-    c.g_at = a.g;
-    c.pz = a.pz;
-    c.pxz = a.pxz;
-    c.pg = a.pg;
-    if (a.x == x) return true;
-    if (j == P.count - 1) return false;
-    const RingVertex& b = P.ring[(j + 1) % RING];
-    const float f = (x - a.x) / (b.x - a.x);
-    const float z = a.z + f * (b.z - a.z);
-    const float g = a.g + f * (b.g - a.g);
+// The profile at x on the segment from `a` toward (bx, bz, bg), a.x <= x < bx (the upper side of
+// a step: the last vertex at or before x, interpolated toward the next), with the prefix to x.
+QM_DEV void segment_at(const Vertex& a, float bx, float bz, float bg, float x, double z0, Cand& c) {
+    if (a.x == x) {
+        c.z_at = a.z;
+        c.g_at = a.g;
+        c.pz = a.pz;
+        c.pxz = a.pxz;
+        c.pg = a.pg;
+        return;
+    }
+    const float f = (x - a.x) / (bx - a.x);
+    const float z = a.z + f * (bz - a.z);
+    const float g = a.g + f * (bg - a.g);
+    const Vertex v = extend(a, x, z, g, z0);
     c.z_at = z;
     c.g_at = g;
-    const double u0 = a.x, u1 = x, width = u1 - u0;
-    const double w0 = (double)a.z - P.z0, w1 = (double)z - P.z0;
-    c.pz += 0.5 * (w0 + w1) * width;
-    c.pxz += width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
-    c.pg += 0.5 * ((double)a.g + (double)g) * width;
-    return true;
+    c.pz = v.pz;
+    c.pxz = v.pxz;
+    c.pg = v.pg;
+}
+
+// The profile at a walk vertex's x just after the walk has handled it: the vertex itself when it
+// was emitted, else the roof covering it (x in (near wall, far wall]).
+QM_DEV void profile_at_walk(const Profile& P, float x, Cand& c) {
+    if (P.head.x == x) segment_at(P.head, x, P.head.z, P.head.g, x, P.z0, c);
+    else segment_at(P.roof_near, P.roof_x1, P.roof_top1, 0.0f, x, P.z0, c);
 }
 
 struct Roof {
@@ -764,7 +849,7 @@ __device__ __noinline__
 #else
 QM_DEV
 #endif
-void state_boundary(const BandTable& B, StateStream& st, bool favourable, P2f Sf, P2f Rf, double z0,
+void state_boundary(const BandTable& B, StateStream& st, const CandLog& log, bool favourable, P2f Sf, P2f Rf, double z0,
                            double L, double pz_L, double pxz_L, double pg_L, const Geometry& direct,
                            float path_ground, float near_source, float source_g, float g_start,
                            uint32_t& failed, float att[BANDS]) {
@@ -774,10 +859,7 @@ void state_boundary(const BandTable& B, StateStream& st, bool favourable, P2f Sf
     if (st.blocking) {
         if (st.has_pending) hull_push(st, Sf, st.pending, failed);
         // The receiver closes the band.
-        Cand r;
-        r.x = Rf.x;
-        r.z = Rf.z;
-        hull_push(st, Sf, r, failed);
+        hull_push(st, Sf, CandRef{Rf.x, Rf.z, -1}, failed);
         --st.hull_n;  // drop the receiver itself
     } else if (st.have_best) {
         st.hull[0] = st.best;
@@ -794,10 +876,12 @@ void state_boundary(const BandTable& B, StateStream& st, bool favourable, P2f Sf
     const double delta = path_difference_d(radius, S, pts, n, R);
     double span = 0.0;
     for (int i = 0; i + 1 < n; ++i) span += dist_d(pts[i], pts[i + 1]) + arc_excess_d(radius, dist_d(pts[i], pts[i + 1]));
-    const Cand& first = st.hull[0];
-    const Cand& last = st.hull[n - 1];
-    const Plane ps = plane_over(0.0, first.x, z0, z0, 0.0, 0.0, first.pz, first.pxz);
-    const Plane pr = plane_over(last.x, L, z0, last.z_at, last.pz, last.pxz, pz_L, pxz_L);
+    const CandRef& first = st.hull[0];
+    const CandRef& last = st.hull[n - 1];
+    const CandData& fd = log.entry[first.id];
+    const CandData& ld = log.entry[last.id];
+    const Plane ps = plane_over(0.0, first.x, z0, z0, 0.0, 0.0, fd.pz, fd.pxz);
+    const Plane pr = plane_over(last.x, L, z0, ld.z_at, ld.pz, ld.pxz, pz_L, pxz_L);
     const Geometry gs = geometry_over(ps, S, pts[0]);
     const Geometry gr = geometry_over(pr, pts[n - 1], R);
     const P2d s_image = mirror(ps, S), r_image = mirror(pr, R);
@@ -817,9 +901,9 @@ void state_boundary(const BandTable& B, StateStream& st, bool favourable, P2f Sf
     if (!any) return;
     const float span_f = (float)span;
     const double ls = (double)first.x;
-    const float g_so = ls < 1e-3 ? g_start : (float)(first.pg / ls);
+    const float g_so = ls < 1e-3 ? g_start : (float)(fd.pg / ls);
     const double lr = L - (double)last.x;
-    const float g_or = lr < 1e-3 ? last.g_at : (float)((pg_L - last.pg) / lr);
+    const float g_or = lr < 1e-3 ? ld.g_at : (float)((pg_L - ld.pg) / lr);
     const float near_so = near_source_factor(gs, g_so, source_g);
     float ground_so[BANDS], ground_or[BANDS];
     ground_bands(B, gs, g_so, favourable ? g_so : near_so, near_so, favourable, ground_so);

@@ -1,8 +1,9 @@
 """One launch of each kernel on a fixed workload, for rocprofv3 (kernel traces and counters).
 
-    python -m quietmap_amd.profile_workload --kernels paint,loudness,aircraft,copy --scale 0.125
+    python -m quietmap_amd.profile_workload --kernels paint,loudness,aircraft,copy
 
-`paint` paints the tile-scale scene at `--scale` (one paint_kernel launch), `loudness` runs
+`paint` paints the central `--window` x `--window` receivers of the tile-scale scene at `--scale`
+in one paint_kernel launch (the defaults are the benchmark's step), `loudness` runs
 zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
 `--aircraft-scale`, and `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
@@ -24,7 +25,8 @@ from .painter import RECEIVER_BYTES, Gpu
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--kernels", default="paint,loudness,aircraft,copy")
-    ap.add_argument("--scale", type=float, default=0.125)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--window", type=int, default=1024)
     ap.add_argument("--cells", type=int, default=16 * 1024 * 1024)
     ap.add_argument("--aircraft-scale", type=float, default=0.05)
     ap.add_argument("--copy-mib", type=int, default=1024)
@@ -33,10 +35,10 @@ def main():
     gpu = Gpu(0)
     if "paint" in kernels:
         sc = scenes.tile_scene(scale=args.scale)
-        nx, ny = sc.grid_shape
+        points, nx, ny = sc.window(args.window)
         n = nx * ny
         square = gpu.upload(sc.scene, sc.sources)
-        d_points = hip.DeviceBuffer.from_array(sc.points)
+        d_points = hip.DeviceBuffer.from_array(points)
         d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
         d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
         gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)

@@ -172,6 +172,14 @@ class SyntheticScene:
         self.name, self.scene, self.sources, self.points = name, scene, sources, points
         self.grid_shape, self.cutoff_m, self.rings = grid_shape, cutoff_m, rings
 
+    def window(self, side):
+        """The central side x side receivers (row-major) and the window's shape (nx, ny)."""
+        gx, gy = self.grid_shape
+        nx, ny = min(side, gx), min(side, gy)
+        x0, y0 = (gx - nx) // 2, (gy - ny) // 2
+        index = ((np.arange(ny)[:, None] + y0) * gx + (np.arange(nx)[None, :] + x0)).reshape(-1)
+        return np.ascontiguousarray(self.points[index]), nx, ny
+
 
 def road_scene(seed=1, receivers=256, spacing=10.0, lines=200, per_line=10, buildings=500):
     """Configuration "plain CNOSSOS road scene": see the module docstring."""
