@@ -36,6 +36,12 @@ EXPORTS = (
     "qm_loudness_device",
     "qm_aircraft_grid",
     "qm_crossings",
+    "qm_mlp_new",
+    "qm_mlp_params",
+    "qm_mlp_free",
+    "qm_mlp_gradient_device",
+    "qm_mlp_train_step_device",
+    "qm_mlp_infer_device",
 )
 RECEIVER_BYTES = 256
 
@@ -70,6 +76,13 @@ def library():
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
         lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, cp, sz]
+        lib.qm_mlp_new.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
+        lib.qm_mlp_params.argtypes = [vp, vp, cp, sz]
+        lib.qm_mlp_free.argtypes = [vp]
+        lib.qm_mlp_free.restype = None
+        lib.qm_mlp_gradient_device.argtypes = [vp, vp, vp, sz, vp, C.POINTER(C.c_float), vp, cp, sz]
+        lib.qm_mlp_train_step_device.argtypes = [vp, vp, vp, sz, C.c_float, C.POINTER(C.c_float), vp, cp, sz]
+        lib.qm_mlp_infer_device.argtypes = [vp, vp, sz, vp, vp, cp, sz]
         _lib = lib
     return _lib
 

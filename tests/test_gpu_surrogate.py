@@ -1,0 +1,110 @@
+"""The surrogate network's HIP kernels (BF16 MFMA forward, backward, Adam) against the FP64
+restatement with the same BF16 rounding points: loss, every layer's weight and bias gradients,
+outputs, one Adam step, and a training run that learns a smooth target."""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from quietmap_amd.painter import Gpu
+
+    g = Gpu(0)
+    yield g
+    g.close()
+
+
+def setup(seed, rows):
+    from quietmap_amd import surrogate
+
+    rng = np.random.default_rng(seed)
+    params = surrogate.init_params(seed)
+    params = (params + rng.normal(0, 0.05, params.size) * (params == 0)).astype(np.float32)
+    x = rng.normal(0, 1, (rows, mlp.SHAPES[0][1])).astype(np.float32)
+    t = rng.normal(0, 1, (rows, mlp.SHAPES[-1][0])).astype(np.float32)
+    return params, x, t
+
+
+def layer_slices():
+    out, k = [], 0
+    for o, i in mlp.SHAPES:
+        out.append((slice(k, k + o * i), slice(k + o * i, k + o * i + o)))
+        k += o * i + o
+    return out
+
+
+@pytest.mark.parametrize("rows", [300, 1024])
+def test_gradients_match_the_oracle(gpu, rows):
+    from quietmap_amd.surrogate import Mlp
+
+    print(f"case: mlp gradient rows={rows}", flush=True)
+    params, x, t = setup(3, rows)
+    net = Mlp(gpu, params, max_batch=rows)
+    loss, grad = net.gradient(x, t)
+    want_loss, want = mlp.loss_and_gradients(params.astype(np.float64), x, t, rounding=True)
+    assert abs(loss - want_loss) <= 1e-4 * want_loss
+    for li, (ws, bs) in enumerate(layer_slices()):
+        for name, sl in (("W", ws), ("b", bs)):
+            ref = want[sl]
+            err = np.linalg.norm(grad[sl] - ref) / max(np.linalg.norm(ref), 1e-30)
+            print(f"layer {li} {name}: relative error {err:.2e}", flush=True)
+            assert err < 2e-3, (li, name, err)
+    net.close()
+
+
+def test_outputs_match_the_oracle(gpu):
+    from quietmap_amd.surrogate import Mlp
+
+    print("case: mlp inference", flush=True)
+    params, x, _ = setup(4, 777)
+    net = Mlp(gpu, params, max_batch=777)
+    y = net.infer(x)
+    want, _ = mlp.forward(params.astype(np.float64), x, rounding=True)
+    err = np.max(np.abs(y - want)) / np.max(np.abs(want))
+    print(f"outputs: relative max error {err:.2e}", flush=True)
+    assert err < 1e-3
+    net.close()
+
+
+def test_adam_step_matches_the_oracle(gpu):
+    from quietmap_amd import hip
+    from quietmap_amd.surrogate import Mlp
+
+    print("case: mlp adam", flush=True)
+    params, x, t = setup(5, 256)
+    net = Mlp(gpu, params, max_batch=256)
+    _, grad = net.gradient(x, t)
+    dx, dt = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer.from_array(t)
+    net.train_step_device(dx.ptr, dt.ptr, 256, 1e-3)
+    got = net.params()
+    want, _, _ = mlp.adam(params.astype(np.float64), grad.astype(np.float64), np.zeros(params.size),
+                          np.zeros(params.size), step=1, lr=1e-3)
+    assert np.max(np.abs(got - want)) < 1e-6
+    net.close()
+
+
+def test_training_learns_a_smooth_target(gpu):
+    from quietmap_amd import hip, surrogate
+    from quietmap_amd.surrogate import Mlp
+
+    print("case: mlp training", flush=True)
+    rng = np.random.default_rng(6)
+    proj = rng.normal(0, 0.4, (40, 8))
+    x = rng.normal(0, 1, (8192, 40)).astype(np.float32)
+    t = (np.sin(x @ proj) + 0.5 * (x[:, :8] ** 2 - 1)).astype(np.float32)
+    net = Mlp(gpu, surrogate.init_params(6), max_batch=1024)
+    dx, dt = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer.from_array(t)
+    first = None
+    for step in range(300):
+        b = step % 8
+        loss = net.train_step_device(dx.ptr + b * 1024 * 40 * 4, dt.ptr + b * 1024 * 8 * 4, 1024, 2e-3,
+                                     want_loss=(step == 0 or step == 299))
+        if step == 0:
+            first = loss
+    print(f"training loss {first:.4f} -> {loss:.4f}", flush=True)
+    assert loss < 0.1 * first
+    net.close()

@@ -1,5 +1,5 @@
 """The C ABI library loads without a GPU and exports every entry point include/qm_paint.h
-declares; the product path refuses to run without it (no CPU fallback)."""
+declares (qm_paint.h, qm_surrogate.h); the product path refuses to run without it (no CPU fallback)."""
 import ctypes
 import os
 import re
@@ -12,8 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared():
-    text = open(os.path.join(ROOT, "include", "qm_paint.h")).read()
-    return sorted(set(re.findall(r"\b(qm_[a-z_]+)\s*\(", text)))
+    names = set()
+    for header in ("qm_paint.h", "qm_surrogate.h"):
+        text = open(os.path.join(ROOT, "include", header)).read()
+        names |= set(re.findall(r"\b(qm_[a-z_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_and_python_agree():
