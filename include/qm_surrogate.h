@@ -53,6 +53,26 @@ int qm_mlp_train_step_device(qm_mlp* mlp, const float* d_inputs, const float* d_
 int qm_mlp_infer_device(qm_mlp* mlp, const float* d_inputs, size_t rows, float* d_outputs, void* stream, char* err,
                         size_t errlen);
 
+/* Training samples: per pair (receiver `d_pair_receiver[i]` of `d_receivers` as
+ * qm_receivers_device makes them, source `d_pair_source[i]` in upload order) the network's two
+ * input rows (calm, then favourable: 2 x QM_MLP_INPUTS floats at d_inputs + 80*i), the exact
+ * path's A_ground + A_dif per band of each state as targets (2 x 8 floats at d_targets + 16*i) and
+ * `d_in_envelope[i]` (1 when the path lies in the network's feature envelope and the exact path
+ * did not fail, else 0). The inputs are those of quietmap_amd/csrc/qm_features.hpp. */
+int qm_surrogate_samples_device(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_receivers,
+                                const uint32_t* d_pair_receiver, const uint32_t* d_pair_source, size_t n_pairs,
+                                float* d_inputs, float* d_targets, uint32_t* d_in_envelope, void* stream, char* err,
+                                size_t errlen);
+
+/* The painted grid with the surrogate: qm_paint_receivers_device's contract (receivers on a
+ * row-major nx x ny grid, every source within `cutoff_m`), with the network's boundary terms for
+ * every path inside its envelope and the exact path for the others. `d_surrogate_paths`
+ * (optional) receives per receiver the paths the network answered. */
+int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square* square, qm_mlp* mlp,
+                                        const qm_receiver* d_receivers, uint32_t nx, uint32_t ny, double cutoff_m,
+                                        double* d_period_energy, uint32_t* d_paths, uint32_t* d_failures,
+                                        uint32_t* d_surrogate_paths, void* stream, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -246,6 +246,31 @@ int qmo_evaluate_pairs(const qm_terrain* terrain, const qm_obstacles* obstacles,
     return 0;
 }
 
+// The boundary terms (A_ground + A_dif per state and band, physics/src/cnossos/diffraction.rs:
+// 64-196) of every pair, `boundary` n_pairs x 2 x 8, with each pair's failure bits.
+int qmo_pair_boundaries(const qm_terrain* terrain, const qm_obstacles* obstacles, const qm_weather* weather,
+                        const qm_point* points, size_t n_points, const qm_source* sources, size_t n_sources,
+                        const uint32_t* pair_point, const uint32_t* pair_source, size_t n_pairs, double* boundary,
+                        uint32_t* failed, int threads) {
+    Scene S{*terrain, *obstacles, *weather};
+    for (size_t i = 0; i < n_pairs; ++i)
+        if (pair_point[i] >= n_points || pair_source[i] >= n_sources) return -1;
+    std::vector<Receiver> receivers(n_points);
+    std::vector<uint32_t> receiver_failed(n_points);
+    for (size_t i = 0; i < n_points; ++i) receiver_failed[i] = (uint32_t)receiver_of(S, points[i], receivers[i]);
+    const int workers = std::max(threads, 1);
+    std::vector<PairScratch> scratch(workers);
+    parallel_for(n_pairs, workers, [&](size_t i, int k) {
+        const uint32_t p = pair_point[i];
+        PairResult r{};
+        failed[i] = receiver_failed[p];
+        if (!failed[i]) failed[i] = (uint32_t)evaluate_pair(S, receivers[p], sources[pair_source[i]], scratch[k], r, nullptr);
+        for (int st = 0; st < 2; ++st)
+            for (int b = 0; b < BANDS; ++b) boundary[(i * 2 + st) * BANDS + b] = failed[i] ? 0.0 : r.boundary[st][b];
+    });
+    return 0;
+}
+
 // The painted grid: per receiver the summed mixed energy per period of every source within
 // `cutoff_m` (horizontal), the paths counted and failures; Lden follows from lden_energy.
 int qmo_paint_points(const qm_terrain* terrain, const qm_obstacles* obstacles, const qm_weather* weather,

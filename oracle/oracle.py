@@ -60,6 +60,8 @@ def _declare(L):
     L.qmo_receivers.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, D, D, D, U32]
     L.qmo_evaluate_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                      C.c_size_t, U32, U32, F32, C.c_size_t, D, U32, D, C.c_int, U64]
+    L.qmo_pair_boundaries.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                      C.c_size_t, U32, U32, C.c_size_t, D, U32, C.c_int]
     L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_size_t, C.c_double, D, U64, U32, C.c_int]
     L.qmo_zwicker.argtypes = [D, C.c_size_t, D]
@@ -281,6 +283,23 @@ def evaluate_pairs(scene: abi.Scene, points, sources, pair_point, pair_source, p
         raise ValueError("oracle: pair index out of range")
     keys = ("vertices", "cells", "cells_occupied", "edges", "crossings", "candidates")
     return dict(sums=sums, failed=failed, point_bands=bands, work=dict(zip(keys, (int(w) for w in work))))
+
+
+def pair_boundaries(scene: abi.Scene, points, sources, pair_point, pair_source, threads=None):
+    """Per pair the boundary terms A_ground + A_dif [2 states][8 bands] (dB) and the failure bits."""
+    points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+    sources = np.ascontiguousarray(sources, abi.SOURCE_DTYPE)
+    pp = np.ascontiguousarray(pair_point, np.uint32)
+    ps = np.ascontiguousarray(pair_source, np.uint32)
+    n = pp.size
+    out = np.zeros((n, 2, 8))
+    failed = np.zeros(n, np.uint32)
+    rc = lib().qmo_pair_boundaries(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), C.byref(scene.weather.c),
+                                   points.ctypes.data, points.size, sources.ctypes.data, sources.size, _p(pp, U32),
+                                   _p(ps, U32), n, _p(out, D), _p(failed, U32), threads or os.cpu_count() or 1)
+    if rc != 0:
+        raise ValueError("oracle: pair index out of range")
+    return out, failed
 
 
 def paint_points(scene: abi.Scene, points, sources, cutoff_m, threads=None):

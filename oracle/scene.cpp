@@ -457,6 +457,8 @@ int evaluate_pair(const Scene& S, const Receiver& receiver, const qm_source& sou
                               1.0);
     const double weight = 1.0 / (d * d * energy(11.0));
     const Transfer& tr = s.terms.transfer;
+    for (int st = 0; st < 2; ++st)
+        for (int b = 0; b < BANDS; ++b) out.boundary[st][b] = s.terms.boundaries[st].attenuation_db[b];
     for (int period = 0; period < PERIODS; ++period)
         for (int b = 0; b < BANDS; ++b) {
             const double power = weight * (double)source.energy[period][b];

@@ -37,6 +37,7 @@ struct PairResult {
     double bands[PERIODS][BANDS];
     double mixed[PERIODS];
     double states[2][PERIODS];
+    double boundary[2][BANDS];  // A_ground + A_dif per state and band
 };
 
 struct PairScratch {

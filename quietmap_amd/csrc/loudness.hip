@@ -13,6 +13,7 @@
 #include <cstdio>
 
 #include "../../include/qm_paint.h"
+#include "qm_internal.hpp"
 
 namespace {
 
@@ -144,6 +145,8 @@ extern "C" int qm_loudness_device(qm_gpu* gpu, const float* d_levels, size_t n, 
     if (n == 0) return 0;
     const size_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffffu) return zfail(err, errlen, "too many cells");
+    const hipError_t d = hipSetDevice(gpu->ordinal);
+    if (d != hipSuccess) return zfail(err, errlen, "hipSetDevice: %s", hipGetErrorString(d));
     hipLaunchKernelGGL(zwicker_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, d_levels, n,
                        cell_stride, band_stride, d_sone);
     const hipError_t e = hipGetLastError();
@@ -155,7 +158,8 @@ extern "C" int qm_loudness(qm_gpu* gpu, const float* levels, size_t n, float* so
     if (!gpu || (n && (!levels || !sone))) return zfail(err, errlen, "null argument");
     if (n == 0) return 0;
     float *d_levels = nullptr, *d_sone = nullptr;
-    hipError_t e = hipMalloc((void**)&d_levels, sizeof(float) * 28 * n);
+    hipError_t e = hipSetDevice(gpu->ordinal);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_levels, sizeof(float) * 28 * n);
     if (e == hipSuccess) e = hipMalloc((void**)&d_sone, sizeof(float) * n);
     if (e == hipSuccess) e = hipMemcpy(d_levels, levels, sizeof(float) * 28 * n, hipMemcpyHostToDevice);
     int rc = 0;

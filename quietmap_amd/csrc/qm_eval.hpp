@@ -12,7 +12,8 @@ struct PathScratch {
     Crossing cross[CROSS_CAP];
     Roof roofs[ROOF_CAP];
     CandLog log;
-    StateStream state[2];
+    Hull hull[2];
+    StateStream* state;  // the two states' scalars (the caller's: registers, scratch or LDS)
 };
 
 // The crest thinning of physics/src/ray.rs:360-385: the open group's best sample.
@@ -44,8 +45,8 @@ QM_DEV float favourable_at(const float* row, float azimuth) {
 QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
     if (!(c.x > 0.0f && c.x < length)) return;
     c.id = -1;
-    state_candidate(w.state, 0, w.log, S, R, c, failed);
-    state_candidate(w.state, 1, w.log, S, R, c, failed);
+    state_candidate(w.state, w.hull, 0, w.log, S, R, c, failed);
+    state_candidate(w.state, w.hull, 1, w.log, S, R, c, failed);
 }
 
 // The x of the first top at or after index i not dropped (+inf when none).
@@ -128,6 +129,32 @@ QM_DEV void crest_vertex(PathScratch& w, const Profile& P, CrestGroup& cg, Tops&
     }
 }
 
+// The path's crossings (exact, FP64) with flag bit 2 on those that never screen it: the receiver's
+// own walls (evaluate.rs:293-297) and, for a source in a building, the walls of every footprint
+// containing it (ray.rs:201-226).
+QM_DEV uint32_t path_crossings(const Scene& sc, const qm_receiver& rcv, const qm_source& src, Crossing* cross, int& n) {
+    n = 0;
+    const uint32_t failed = find_crossings(sc.obstacles, src.x_m, src.y_m, rcv.x_m, rcv.y_m, cross, n);
+    if (failed & QM_FAILED_NOT_READ) return failed;
+    for (int i = 0; i < n; ++i) {
+        cross[i].flags &= 3u;
+        if (rcv.own_footprint != 0 && cross[i].footprint == rcv.own_footprint) cross[i].flags |= 4u;
+    }
+    if (src.footprint_id != 0) {
+        for (int i = 0; i < n; ++i) {
+            if (cross[i].flags & 4u) continue;
+            const uint64_t id = cross[i].footprint;
+            int count = 0;
+            for (int j = 0; j < n; ++j)
+                if (!(cross[j].flags & 4u) && (cross[j].flags & 2u) && cross[j].footprint == id) ++count;
+            if ((count & 1) || id == src.footprint_id) cross[i].flags |= 8u;
+        }
+        for (int i = 0; i < n; ++i)
+            if (cross[i].flags & 8u) cross[i].flags = (cross[i].flags & ~8u) | 4u;
+    }
+    return failed;
+}
+
 // A ray's foliage depth in one state (foliage.rs:36-64), streamed over the terrain samples.
 struct Foliage {
     float depth[2];
@@ -189,6 +216,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         for (int b = 0; b < BANDS; ++b) out.bands[p][b] = 0.0f;
         out.states[0][p] = out.states[1][p] = 0.0f;
     }
+    for (int b = 0; b < BANDS; ++b) out.att[0][b] = out.att[1][b] = 0.0f;
     out.failed = failed;
     if (failed) return;
     const qm_terrain& T = sc.terrain;
@@ -200,29 +228,11 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const double length_d = hypot(sx - rx, sy - ry);
     const float H = (float)fmax(length_d, 1.0);
 
-    // Crossings, the receiver's own walls dropped (evaluate.rs:293-297).
     int n = 0;
-    failed |= find_crossings(sc.obstacles, sx, sy, rx, ry, w.cross, n);
+    failed |= path_crossings(sc, rcv, src, w.cross, n);
     if (failed & QM_FAILED_NOT_READ) {
         out.failed = failed;
         return;
-    }
-    for (int i = 0; i < n; ++i) {
-        w.cross[i].flags &= 3u;
-        if (rcv.own_footprint != 0 && w.cross[i].footprint == rcv.own_footprint) w.cross[i].flags |= 4u;
-    }
-    // Footprints containing a building's source never screen it (ray.rs:201-226).
-    if (src.footprint_id != 0) {
-        for (int i = 0; i < n; ++i) {
-            if (w.cross[i].flags & 4u) continue;
-            const uint64_t id = w.cross[i].footprint;
-            int count = 0;
-            for (int j = 0; j < n; ++j)
-                if (!(w.cross[j].flags & 4u) && (w.cross[j].flags & 2u) && w.cross[j].footprint == id) ++count;
-            if ((count & 1) || id == src.footprint_id) w.cross[i].flags |= 8u;
-        }
-        for (int i = 0; i < n; ++i)
-            if (w.cross[i].flags & 8u) w.cross[i].flags = (w.cross[i].flags & ~8u) | 4u;
     }
 
     // The ends (tiles/src/terrain.rs:342-385 at both ends of popup/src/scene.rs:242-309).
@@ -324,8 +334,10 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         const float direct = sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt));
         w.state[0].radius = 0.0f;
         w.state[1].radius = fmaxf(1000.0f, 8.0f * direct);
-        w.state[0].line = state_line(w.state[0].radius, S, R);
-        w.state[1].line = state_line(w.state[1].radius, S, R);
+        for (int s = 0; s < 2; ++s) {
+            w.state[s].line = state_line(w.state[s].radius, S, R);
+            w.state[s].arc_sr = arc_excess_f(w.state[s].radius, w.state[s].line.chord);
+        }
         for (int s = 0; s < 2; ++s) {
             w.state[s].blocking = false;
             w.state[s].has_pending = false;
@@ -448,8 +460,12 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float near_source = near_source_factor(direct, path_ground, source_g);
     float att[2][BANDS];
     for (int s = 0; s < 2; ++s)
-        state_boundary(B, w.state[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
+        state_boundary(B, w.state[s], w.hull[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
                        near_source, source_g, s0.g, failed, att[s]);
+    for (int b = 0; b < BANDS; ++b) {
+        out.att[0][b] = att[0][b];
+        out.att[1][b] = att[1][b];
+    }
 
     // ray.rs:166-190: foliage, air at the slant distance, the period mix; evaluate.rs:119-152.
     const float slant = fmaxf(sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt)), 1.0f);

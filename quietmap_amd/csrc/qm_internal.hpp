@@ -2,8 +2,9 @@
 #pragma once
 
 #include <mutex>
+#include <vector>
 
-#include "qm_device.hpp"
+#include "qm_path.hpp"
 
 struct qm_gpu {
     int ordinal;
@@ -11,3 +12,29 @@ struct qm_gpu {
     qm::BandTable bands;
     std::mutex lock;
 };
+
+// Point sources bucketed on a square grid (device pointers): bucket (kx, ky) holds sources
+// bucket_first[ky * buckets_x + kx] .. bucket_first[ky * buckets_x + kx + 1] of `sources`.
+struct SourceGrid {
+    const qm_source* sources;
+    const uint32_t* bucket_first;
+    double origin_x, origin_y, bucket_m;
+    uint32_t buckets_x, buckets_y;
+};
+
+struct qm_square {
+    qm_gpu* gpu;
+    qm::Scene scene;  // device pointers
+    SourceGrid grid;  // device pointers
+    size_t n_sources;
+    qm_source* d_sources_by_input;  // input order (for pairs)
+    std::vector<void*> allocations;
+    size_t bytes;
+};
+
+struct qm_mlp;
+namespace qm {
+// The surrogate network's outputs (rows x 8, FP32) of BF16 input rows (rows x 40) on `stream`.
+int mlp_infer_rows(qm_mlp* mlp, const uint16_t* d_rows, size_t rows, float* d_outputs, hipStream_t stream, char* err,
+                   size_t errlen);
+}  // namespace qm

@@ -428,17 +428,7 @@ struct CandRef {
     int id;
 };
 
-// physics/src/cnossos/rubber_band.rs:182-193, FP32 and FP64 forms.
-QM_DEV float triangle_excess_f(P2f s, P2f o, P2f r) {
-    const float ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
-    const float lu = fsqrt(ux * ux + uz * uz), lv = fsqrt(vx * vx + vz * vz);
-    const float wx = ux + vx, wz = uz + vz;
-    const float lw = fsqrt(wx * wx + wz * wz);
-    const float dot = ux * vx + uz * vz;
-    if (dot <= 0.0f) return lu + lv - lw;
-    const float cross = ux * vz - uz * vx;
-    return fdiv(2.0f * cross * cross, (lu * lv + dot) * (lu + lv + lw));
-}
+// physics/src/cnossos/rubber_band.rs:182-193 (FP64; the FP32 form with its legs is below).
 QM_DEV double triangle_excess_d(P2d s, P2d o, P2d r) {
     const double ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
     const double lu = sqrt(ux * ux + uz * uz), lv = sqrt(vx * vx + vz * vz);
@@ -475,20 +465,17 @@ QM_DEV double arc_excess_d(double radius, double chord) {
     return 2.0 * radius * x * x2 * (1.0 / 6.0 + x2 * (3.0 / 40.0 + x2 * (5.0 / 112.0 + x2 * (35.0 / 1152.0))));
 }
 
-// rubber_band.rs:86-121 path_difference with one point (FP32, the candidates' screening).
-QM_DEV float single_path_difference_f(float radius, P2f from, P2f o, P2f to) {
-    const float chords = triangle_excess_f(from, o, to);
-    const float arcs = arc_excess_f(radius, dist_f(from, o)) + arc_excess_f(radius, dist_f(o, to)) -
-                       arc_excess_f(radius, dist_f(from, to));
-    const float excess = chords + arcs;
-    const float chord = from.z + fdiv((to.z - from.z) * (o.x - from.x), to.x - from.x);
-    if (o.z >= chord) return excess;
-    if (radius <= 0.0f) return -excess;
-    const P2f a{o.x, chord};
-    return 2.0f * triangle_excess_f(from, a, to) - triangle_excess_f(from, o, to) +
-           2.0f * arc_excess_f(radius, dist_f(from, a)) + 2.0f * arc_excess_f(radius, dist_f(a, to)) -
-           arc_excess_f(radius, dist_f(from, o)) - arc_excess_f(radius, dist_f(o, to)) -
-           arc_excess_f(radius, dist_f(from, to));
+// triangle_excess_f with the two legs' lengths |so| and |or| it forms.
+QM_DEV float triangle_excess_legs_f(P2f s, P2f o, P2f r, float& lu, float& lv) {
+    const float ux = o.x - s.x, uz = o.z - s.z, vx = r.x - o.x, vz = r.z - o.z;
+    lu = fsqrt(ux * ux + uz * uz);
+    lv = fsqrt(vx * vx + vz * vz);
+    const float wx = ux + vx, wz = uz + vz;
+    const float lw = fsqrt(wx * wx + wz * wz);
+    const float dot = ux * vx + uz * vz;
+    if (dot <= 0.0f) return lu + lv - lw;
+    const float cross = ux * vz - uz * vx;
+    return fdiv(2.0f * cross * cross, (lu * lv + dot) * (lu + lv + lw));
 }
 
 // rubber_band.rs:86-121 path_difference over n points, FP64 (the delta of the boundary terms).
@@ -562,6 +549,23 @@ QM_DEV float line_altitude(const StateLine& L, float x) {
              L.along_per_x;
     }
     return z;
+}
+
+// rubber_band.rs:86-121 path_difference with one point o (FP32, the candidates' screening) for
+// a state's line S -> R. The excess over the chord is the triangle excess plus the arcs' excesses
+// over the legs; below the straight chord in the favourable state it is mirrored about the chord
+// point A = (o.x, chord(o.x)), whose legs lie on the chord itself (T(S, A, R) = 0, |SA| = along,
+// |AR| = |SR| - along), so the arcs come from lengths the line already holds.
+QM_DEV float single_path_difference_f(const StateLine& L, float radius, float arc_sr, P2f from, P2f o, P2f to) {
+    float lu, lv;
+    const float chords = triangle_excess_legs_f(from, o, to, lu, lv);
+    const float excess = chords + arc_excess_f(radius, lu) + arc_excess_f(radius, lv) - arc_sr;
+    const float dx = o.x - L.sx;
+    const float chord = L.sz + L.slope * dx;
+    if (o.z >= chord) return excess;
+    if (radius <= 0.0f) return -excess;
+    const float along = dx * L.along_per_x;
+    return 2.0f * (arc_excess_f(radius, along) + arc_excess_f(radius, L.chord - along) - arc_sr) - excess;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -678,14 +682,19 @@ QM_DEV float ground_split_db(float ground_db, float mirrored_db, float direct_db
 struct StateStream {
     StateLine line;  // the state's ray from S to R
     float radius;  // 0 calm; the favourable arc's radius
+    float arc_sr;  // the arc's excess over the whole chord S -> R
     bool blocking;
     bool has_pending;
     bool have_best;
-    int hull_n;    // entries in `hull` after the source
+    int hull_n;    // entries of the state's hull (kept apart, HULL_CAP CandRefs) after the source
     float best_delta;
     CandRef pending;
     CandRef best;
-    CandRef hull[HULL_CAP];
+};
+
+// A state's hull: the diffraction points of its rubber band so far, source excluded.
+struct Hull {
+    CandRef e[HULL_CAP];
 };
 
 // The candidate log of one path: the data of every candidate a state has kept.
@@ -696,11 +705,11 @@ struct CandLog {
 
 // Drops the entries no state refers to any more and renumbers the rest in order (the log is full;
 // at most 2*HULL_CAP + 4 entries are live, so this frees at least 12).
-__device__ __noinline__ void log_compact(CandLog& log, StateStream* states) {
+__device__ __noinline__ void log_compact(CandLog& log, StateStream* states, Hull* hulls) {
     uint64_t live = 0;
     for (int s = 0; s < 2; ++s) {
         const StateStream& st = states[s];
-        for (int i = 0; i < st.hull_n; ++i) live |= 1ull << st.hull[i].id;
+        for (int i = 0; i < st.hull_n; ++i) live |= 1ull << hulls[s].e[i].id;
         if (st.has_pending) live |= 1ull << st.pending.id;
         if (st.have_best && !st.blocking) live |= 1ull << st.best.id;
     }
@@ -710,16 +719,17 @@ __device__ __noinline__ void log_compact(CandLog& log, StateStream* states) {
     log.n = next;
     for (int s = 0; s < 2; ++s) {
         StateStream& st = states[s];
-        for (int i = 0; i < st.hull_n; ++i) st.hull[i].id = __popcll(live & ((1ull << st.hull[i].id) - 1ull));
+        for (int i = 0; i < st.hull_n; ++i)
+            hulls[s].e[i].id = __popcll(live & ((1ull << hulls[s].e[i].id) - 1ull));
         if (st.has_pending) st.pending.id = __popcll(live & ((1ull << st.pending.id) - 1ull));
         if (st.have_best && !st.blocking) st.best.id = __popcll(live & ((1ull << st.best.id) - 1ull));
     }
 }
 
 // The candidate as a state keeps it, logging its data on first use (shared by both states).
-QM_DEV CandRef keep(CandLog& log, StateStream* states, Cand& c) {
+QM_DEV CandRef keep(CandLog& log, StateStream* states, Hull* hulls, Cand& c) {
     if (c.id < 0) {
-        if (log.n >= LOG_CAP) log_compact(log, states);
+        if (log.n >= LOG_CAP) log_compact(log, states, hulls);
         c.id = log.n++;
         CandData& d = log.entry[c.id];
         d.z_at = c.z_at;
@@ -731,10 +741,10 @@ QM_DEV CandRef keep(CandLog& log, StateStream* states, Cand& c) {
     return CandRef{c.x, c.z, c.id};
 }
 
-QM_DEV void hull_push(StateStream& st, P2f S, const CandRef& c, uint32_t& failed) {
+QM_DEV void hull_push(StateStream& st, Hull& hull, P2f S, const CandRef& c, uint32_t& failed) {
     while (st.hull_n >= 1) {
-        const P2f a = st.hull_n >= 2 ? P2f{st.hull[st.hull_n - 2].x, st.hull[st.hull_n - 2].z} : S;
-        const P2f b{st.hull[st.hull_n - 1].x, st.hull[st.hull_n - 1].z};
+        const P2f a = st.hull_n >= 2 ? P2f{hull.e[st.hull_n - 2].x, hull.e[st.hull_n - 2].z} : S;
+        const P2f b{hull.e[st.hull_n - 1].x, hull.e[st.hull_n - 1].z};
         const P2f p{c.x, c.z};
         const float ray_at_b = a.z + fdiv((p.z - a.z) * (b.x - a.x), p.x - a.x) + sag_f(st.radius, a, p, b.x - a.x);
         if (b.z <= ray_at_b) --st.hull_n;
@@ -744,29 +754,30 @@ QM_DEV void hull_push(StateStream& st, P2f S, const CandRef& c, uint32_t& failed
         failed |= QM_FAILED_CAPACITY | QM_DETAIL_HULL;
         return;
     }
-    st.hull[st.hull_n++] = c;
+    hull.e[st.hull_n++] = c;
 }
 
 // One candidate (in order of distance) into state `s`'s stream.
-QM_DEV void state_candidate(StateStream* states, int s, CandLog& log, P2f S, P2f R, Cand& c, uint32_t& failed) {
+QM_DEV void state_candidate(StateStream* states, Hull* hulls, int s, CandLog& log, P2f S, P2f R, Cand& c,
+                            uint32_t& failed) {
     StateStream& st = states[s];
     const P2f p{c.x, c.z};
     if (p.z > line_altitude(st.line, p.x)) {
         if (st.has_pending && st.pending.x == p.x) {
-            if (p.z > st.pending.z) st.pending = keep(log, states, c);
+            if (p.z > st.pending.z) st.pending = keep(log, states, hulls, c);
         } else {
-            if (st.has_pending) hull_push(st, S, st.pending, failed);
-            st.pending = keep(log, states, c);
+            if (st.has_pending) hull_push(st, hulls[s], S, st.pending, failed);
+            st.pending = keep(log, states, hulls, c);
             st.has_pending = true;
         }
         st.blocking = true;
         return;
     }
     if (st.blocking) return;  // a blocked state's unblocked-best is never read
-    const float delta = single_path_difference_f(st.radius, S, p, R);
+    const float delta = single_path_difference_f(st.line, st.radius, st.arc_sr, S, p, R);
     if (!st.have_best || delta > st.best_delta) {
         st.best_delta = delta;
-        st.best = keep(log, states, c);
+        st.best = keep(log, states, hulls, c);
         st.have_best = true;
     }
 }
@@ -839,6 +850,7 @@ struct Roof {
 struct PathResult {
     float bands[PERIODS][BANDS];  // received A-weighted energy per period and band (weighted)
     float states[2][PERIODS];     // calm and favourable alone, per period
+    float att[2][BANDS];          // A_ground + A_dif per state and band (the boundary terms), dB
     uint32_t failed;
 };
 
@@ -849,7 +861,7 @@ __device__ __noinline__
 #else
 QM_DEV
 #endif
-void state_boundary(const BandTable& B, StateStream& st, const CandLog& log, bool favourable, P2f Sf, P2f Rf, double z0,
+void state_boundary(const BandTable& B, StateStream& st, Hull& hull, const CandLog& log, bool favourable, P2f Sf, P2f Rf, double z0,
                            double L, double pz_L, double pxz_L, double pg_L, const Geometry& direct,
                            float path_ground, float near_source, float source_g, float g_start,
                            uint32_t& failed, float att[BANDS]) {
@@ -857,12 +869,12 @@ void state_boundary(const BandTable& B, StateStream& st, const CandLog& log, boo
     ground_bands(B, direct, path_ground, favourable ? path_ground : near_source, near_source, favourable, att);
     // The diffraction points.
     if (st.blocking) {
-        if (st.has_pending) hull_push(st, Sf, st.pending, failed);
+        if (st.has_pending) hull_push(st, hull, Sf, st.pending, failed);
         // The receiver closes the band.
-        hull_push(st, Sf, CandRef{Rf.x, Rf.z, -1}, failed);
+        hull_push(st, hull, Sf, CandRef{Rf.x, Rf.z, -1}, failed);
         --st.hull_n;  // drop the receiver itself
     } else if (st.have_best) {
-        st.hull[0] = st.best;
+        hull.e[0] = st.best;
         st.hull_n = 1;
     } else {
         return;  // no candidate at all
@@ -870,14 +882,14 @@ void state_boundary(const BandTable& B, StateStream& st, const CandLog& log, boo
     const int n = st.hull_n;
     if (n <= 0) return;
     P2d pts[HULL_CAP];
-    for (int i = 0; i < n; ++i) pts[i] = P2d{(double)st.hull[i].x, (double)st.hull[i].z};
+    for (int i = 0; i < n; ++i) pts[i] = P2d{(double)hull.e[i].x, (double)hull.e[i].z};
     const P2d S{(double)Sf.x, (double)Sf.z}, R{L, (double)Rf.z};
     const double radius = favourable ? (double)st.radius : 0.0;
     const double delta = path_difference_d(radius, S, pts, n, R);
     double span = 0.0;
     for (int i = 0; i + 1 < n; ++i) span += dist_d(pts[i], pts[i + 1]) + arc_excess_d(radius, dist_d(pts[i], pts[i + 1]));
-    const CandRef& first = st.hull[0];
-    const CandRef& last = st.hull[n - 1];
+    const CandRef& first = hull.e[0];
+    const CandRef& last = hull.e[n - 1];
     const CandData& fd = log.entry[first.id];
     const CandData& ld = log.entry[last.id];
     const Plane ps = plane_over(0.0, first.x, z0, z0, 0.0, 0.0, fd.pz, fd.pxz);

@@ -173,6 +173,94 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
     }
 }
 
+// Inference, all five layers in one workgroup per 64 rows: the activations stay in LDS (two
+// BF16 buffers of 64 x 264), each of the four waves computes 64 output columns of a hidden layer
+// as 2x2 blocks of 32x32 MFMA accumulators with its weights read straight from L2 (16 bytes of
+// one column's K run per lane), and GELU is taken as z * sigmoid(2u), the tanh form's identity.
+constexpr int INFER_ROWS = 64;
+constexpr int INFER_PITCH = QM_MLP_HIDDEN + 8;
+
+__device__ __forceinline__ float gelu_infer(float z) {
+    const float u = GELU_C * (z + GELU_A * z * z * z);
+    return z / (1.0f + __expf(-2.0f * u));
+}
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restrict__ x, size_t rows,
+                                                        const uint16_t* __restrict__ wb, const float* __restrict__ w,
+                                                        float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][INFER_ROWS][INFER_PITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t row0 = (size_t)blockIdx.x * INFER_ROWS;
+    constexpr int K0 = 48;  // the 40 inputs padded to three MFMA K steps
+    for (int e = tid; e < INFER_ROWS * K0; e += 256) {
+        const int r = e / K0, k = e % K0;
+        act[0][r][k] = (k < QM_MLP_INPUTS && row0 + r < rows) ? x[(row0 + r) * QM_MLP_INPUTS + k] : (uint16_t)0;
+    }
+    __syncthreads();
+    int cur = 0;
+    const int lr = lane & 31, lk = 8 * (lane >> 5);
+#pragma unroll 1
+    for (int l = 0; l < LAYERS - 1; ++l) {
+        const int K = l == 0 ? K0 : QM_MLP_HIDDEN;
+        const int fan_in = FAN_IN[l];
+        const uint16_t* W = wb + LAYOUT.w[l];
+        const float* bias = w + LAYOUT.b[l];
+        const int c0 = wave * 64;
+        f32x16_t acc[2][2] = {};
+#pragma unroll 4
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            bf16x8_t a[2], b[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+                a[rb] = *reinterpret_cast<const bf16x8_t*>(&act[cur][rb * 32 + lr][k0 + lk]);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+                b[cb] = *reinterpret_cast<const bf16x8_t*>(W + (size_t)(c0 + cb * 32 + lr) * fan_in + k0 + lk);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+        }
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            const int col = c0 + cb * 32 + lr;
+            const float bc = bias[col];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = rb * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    act[cur ^ 1][row][col] = bf16_bits(gelu_infer(acc[rb][cb][r] + bc));
+                }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // The 8 outputs: waves 0 and 1 each take 32 rows over one 32-column block (8 used).
+    if (wave < 2) {
+        const uint16_t* W = wb + LAYOUT.w[LAYERS - 1];
+        const float* bias = w + LAYOUT.b[LAYERS - 1];
+        f32x16_t acc = {};
+        const bf16x8_t zero = {};
+#pragma unroll 4
+        for (int k0 = 0; k0 < QM_MLP_HIDDEN; k0 += 16) {
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[cur][wave * 32 + lr][k0 + lk]);
+            const bf16x8_t b =
+                lr < QM_MLP_OUTPUTS ? *reinterpret_cast<const bf16x8_t*>(W + (size_t)lr * QM_MLP_HIDDEN + k0 + lk) : zero;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        if (lr < QM_MLP_OUTPUTS) {
+            const float bc = bias[lr];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const size_t row = row0 + wave * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                if (row < rows) out[row * QM_MLP_OUTPUTS + lr] = acc[r] + bc;
+            }
+        }
+    }
+}
+
 __global__ void to_bf16_kernel(const float* __restrict__ x, size_t n, uint16_t* __restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = bf16_bits(x[i]);
@@ -497,6 +585,17 @@ extern "C" int qm_mlp_train_step_device(qm_mlp* p, const float* d_inputs, const 
     return 0;
 }
 
+int qm::mlp_infer_rows(qm_mlp* p, const uint16_t* d_rows, size_t rows, float* d_outputs, hipStream_t s, char* err,
+                       size_t errlen) {
+    if (rows == 0) return 0;
+    const size_t blocks = (rows + INFER_ROWS - 1) / INFER_ROWS;
+    if (blocks > 0x7fffffffu) return fail(err, errlen, "too many rows");
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_rows, rows, p->wb, p->w, d_outputs);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(err, errlen, "mlp_infer_kernel: %s", hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int qm_mlp_infer_device(qm_mlp* p, const float* d_inputs, size_t rows, float* d_outputs, void* stream,
                                    char* err, size_t errlen) {
     if (!p || (rows && (!d_inputs || !d_outputs))) return fail(err, errlen, "null argument");
@@ -509,22 +608,6 @@ extern "C" int qm_mlp_infer_device(qm_mlp* p, const float* d_inputs, size_t rows
         hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(nx, 256)), dim3(256), 0, s, d_inputs, nx, p->xb);
         e = hipGetLastError();
     }
-    const uint16_t* in = p->xb;
-    for (int l = 0; l < LAYERS && e == hipSuccess; ++l) {
-        const int K = FAN_IN[l], N = FAN_OUT[l];
-        GemmArgs g = gemm((int)rows, N, K, in, K, 1, p->wb + LAYOUT.w[l], 1, K,
-                          l + 1 < LAYERS ? INFER_HIDDEN : FORWARD_OUTPUT);
-        g.bias = p->w + LAYOUT.b[l];
-        if (l + 1 < LAYERS) {
-            g.act = p->a[l];
-            g.ldact = N;
-            in = p->a[l];
-        } else {
-            g.Z = d_outputs;
-            g.ldz = N;
-        }
-        e = launch_gemm(g, 1, s);
-    }
     if (e != hipSuccess) return fail(err, errlen, "qm_mlp_infer_device: %s", hipGetErrorString(e));
-    return 0;
+    return qm::mlp_infer_rows(p, p->xb, rows, d_outputs, s, err, errlen);
 }

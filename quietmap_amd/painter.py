@@ -42,6 +42,8 @@ EXPORTS = (
     "qm_mlp_gradient_device",
     "qm_mlp_train_step_device",
     "qm_mlp_infer_device",
+    "qm_surrogate_samples_device",
+    "qm_surrogate_paint_receivers_device",
 )
 RECEIVER_BYTES = 256
 
@@ -83,6 +85,8 @@ def library():
         lib.qm_mlp_gradient_device.argtypes = [vp, vp, vp, sz, vp, C.POINTER(C.c_float), vp, cp, sz]
         lib.qm_mlp_train_step_device.argtypes = [vp, vp, vp, sz, C.c_float, C.POINTER(C.c_float), vp, cp, sz]
         lib.qm_mlp_infer_device.argtypes = [vp, vp, sz, vp, vp, cp, sz]
+        lib.qm_surrogate_samples_device.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
+        lib.qm_surrogate_paint_receivers_device.argtypes = [vp, vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, vp, cp, sz]
         _lib = lib
     return _lib
 
