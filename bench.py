@@ -38,6 +38,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loudness", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--loudness-cells", type=int, default=16 * 1024 * 1024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -97,6 +98,48 @@ def main():
     kernel_s = sum(times) / len(times)
     value = paths / kernel_s
     cells = n / kernel_s
+
+    # The surrogate leg: the same window painted with the network inside its envelope.
+    sur = None
+    params_file = os.path.join(ROOT, "quietmap_amd", "surrogate_params.npy")
+    if not args.no_surrogate and os.path.exists(params_file):
+        from quietmap_amd import surrogate
+
+        net = surrogate.Mlp(gpu, np.load(params_file), max_batch=1)
+        d_learned = hip.DeviceBuffer(n * 4)
+        d_energy_s = hip.DeviceBuffer(n * 3 * 8)
+
+        def surrogate_step():
+            surrogate.paint_receivers_device(gpu, square, net, d_receivers.ptr, nx, ny, sc.cutoff_m, d_energy_s.ptr,
+                                             d_paths.ptr, d_failures.ptr, d_learned.ptr, stream.handle)
+
+        surrogate_step()
+        stream.synchronize()
+        st = []
+        for k in range(args.steps):
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            surrogate_step()
+            e1.record(stream)
+            e1.synchronize()
+            st.append(e1.seconds_since(e0))
+            print(f"[bench] surrogate step {k + 1}/{args.steps}: {st[-1]:.3f} s", file=sys.stderr, flush=True)
+        s_time = sum(st) / len(st)
+        s_paths = int(d_paths.to_array(np.uint32, n).astype(np.int64).sum())
+        learned = int(d_learned.to_array(np.uint32, n).astype(np.int64).sum())
+        lden_exact = lden_db(energy)
+        lden_sur = lden_db(d_energy_s.to_array(np.float64, (n, 3)))
+        err_s = np.abs(lden_sur - lden_exact)
+        err_s = err_s[np.isfinite(err_s)]
+        sur = {"value": s_paths / s_time, "unit": "paths/s", "ms_per_step": s_time * 1e3,
+               "speedup_vs_exact": (s_paths / s_time) / value, "network_paths_fraction": learned / max(s_paths, 1),
+               "lden_vs_exact_db": {"mean": float(err_s.mean()), "p99": float(np.quantile(err_s, 0.99)),
+                                    "p999": float(np.quantile(err_s, 0.999)), "max": float(err_s.max())},
+               "mlp_flop_per_inference": 417792}
+        net.close()
+        print(f"[bench] surrogate {sur['value']:.3e} paths/s ({sur['speedup_vs_exact']:.2f}x exact), Lden vs exact "
+              f"mean {sur['lden_vs_exact_db']['mean']:.3f} max {sur['lden_vs_exact_db']['max']:.3f} dB",
+              file=sys.stderr, flush=True)
 
     # The loudness leg: ISO 532-1 on band-major planes of 28 third-octave levels per cell.
     loud = None
@@ -223,6 +266,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_fp64_oracle": parity,
         "loudness": loud,
+        "surrogate": sur,
         "gpu": gpu.name,
     }
     print(json.dumps(line), flush=True)

@@ -271,6 +271,80 @@ int qmo_pair_boundaries(const qm_terrain* terrain, const qm_obstacles* obstacles
     return 0;
 }
 
+// physics/src/line.rs line_quadrature_nodes for a piece from `start` to `end` (receiver-relative,
+// metres), the skyline handing every given arc (lo, hi, nearest; n_arcs x 3) to every wide bucket.
+// info: length, subtended in-plane angle, divergence factor. Returns the node count (nodes beyond
+// `capacity` not written) or -1 for a degenerate piece.
+long qmo_line_nodes(const double* start, const double* end, const double* arcs, size_t n_arcs, size_t capacity,
+                    double* along, double* weight, uint8_t* obstacles, double* info) {
+    LineGeometry g;
+    if (!line_geometry(start, end, g)) return -1;
+    std::vector<LineNode> nodes;
+    line_quadrature_nodes(
+        g,
+        [&](double, double, double, const std::function<void(const SkylineArc&)>& visit) {
+            for (size_t i = 0; i < n_arcs; ++i) visit(SkylineArc{arcs[3 * i], arcs[3 * i + 1], arcs[3 * i + 2]});
+        },
+        nodes);
+    for (size_t i = 0; i < nodes.size() && i < capacity; ++i) {
+        along[i] = nodes[i].along_m;
+        weight[i] = nodes[i].weight_rad;
+        obstacles[i] = nodes[i].obstacles_on_ray ? 1 : 0;
+    }
+    info[0] = g.length_m;
+    info[1] = line_subtended(g);
+    info[2] = line_divergence_factor(g);
+    return (long)nodes.size();
+}
+
+// The skyline arcs (popup/src/obstacles/skyline.rs) of a receiver at (x, y) over a wedge.
+long qmo_skyline_arcs(const qm_obstacles* obstacles, double x, double y, double lo, double hi, double radius,
+                      double floor_m, size_t capacity, double* out) {
+    const double origin[2] = {x, y};
+    size_t n = 0;
+    skyline_arcs(*obstacles, origin, lo, hi, radius, floor_m, [&](const SkylineArc& a) {
+        if (n < capacity) {
+            out[3 * n] = a.lo_rad;
+            out[3 * n + 1] = a.hi_rad;
+            out[3 * n + 2] = a.nearest_m;
+        }
+        ++n;
+    });
+    return (long)n;
+}
+
+// Line pairs as qmo_evaluate_pairs: per pair the 9 sums, the failure bits and the rays (nodes).
+int qmo_evaluate_line_pairs(const qm_terrain* terrain, const qm_obstacles* obstacles, const qm_weather* weather,
+                            const qm_point* points, size_t n_points, const qm_line_piece* pieces, size_t n_pieces,
+                            const uint32_t* pair_point, const uint32_t* pair_piece, size_t n_pairs, double* sums,
+                            uint32_t* failed, uint32_t* rays, int threads) {
+    Scene S{*terrain, *obstacles, *weather};
+    for (size_t i = 0; i < n_pairs; ++i)
+        if (pair_point[i] >= n_points || pair_piece[i] >= n_pieces) return -1;
+    std::vector<Receiver> receivers(n_points);
+    std::vector<uint32_t> receiver_failed(n_points);
+    for (size_t i = 0; i < n_points; ++i) receiver_failed[i] = (uint32_t)receiver_of(S, points[i], receivers[i]);
+    const int workers = std::max(threads, 1);
+    std::vector<PairScratch> scratch(workers);
+    std::vector<std::vector<LineNode>> nodes(workers);
+    parallel_for(n_pairs, workers, [&](size_t i, int k) {
+        const uint32_t p = pair_point[i];
+        PairResult r{};
+        failed[i] = receiver_failed[p];
+        if (!failed[i])
+            failed[i] = (uint32_t)evaluate_line_pair(S, receivers[p], pieces[pair_piece[i]], scratch[k], r, nodes[k],
+                                                     nullptr);
+        rays[i] = failed[i] ? 0u : (uint32_t)nodes[k].size();
+        double* out = sums + i * QM_PAIR_SUMS;
+        for (int q = 0; q < PERIODS; ++q) {
+            out[q] = failed[i] ? 0.0 : r.mixed[q];
+            out[3 + q] = failed[i] ? 0.0 : r.states[0][q];
+            out[6 + q] = failed[i] ? 0.0 : r.states[1][q];
+        }
+    });
+    return 0;
+}
+
 // The painted grid: per receiver the summed mixed energy per period of every source within
 // `cutoff_m` (horizontal), the paths counted and failures; Lden follows from lden_energy.
 int qmo_paint_points(const qm_terrain* terrain, const qm_obstacles* obstacles, const qm_weather* weather,

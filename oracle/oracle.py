@@ -60,6 +60,13 @@ def _declare(L):
     L.qmo_receivers.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, D, D, D, U32]
     L.qmo_evaluate_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                      C.c_size_t, U32, U32, F32, C.c_size_t, D, U32, D, C.c_int, U64]
+    L.qmo_line_nodes.argtypes = [D, D, D, C.c_size_t, C.c_size_t, D, D, U8, D]
+    L.qmo_line_nodes.restype = C.c_long
+    L.qmo_skyline_arcs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_size_t, D]
+    L.qmo_skyline_arcs.restype = C.c_long
+    L.qmo_evaluate_line_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                          C.c_size_t, U32, U32, C.c_size_t, D, U32, U32, C.c_int]
     L.qmo_pair_boundaries.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                       C.c_size_t, U32, U32, C.c_size_t, D, U32, C.c_int]
     L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
@@ -300,6 +307,47 @@ def pair_boundaries(scene: abi.Scene, points, sources, pair_point, pair_source, 
     if rc != 0:
         raise ValueError("oracle: pair index out of range")
     return out, failed
+
+
+def line_nodes(start, end, arcs=(), capacity=4096):
+    """The quadrature nodes of a piece (receiver-relative ends), every arc (lo, hi, nearest) handed
+    to each wide bucket: dict(along, weight, obstacles, length, subtended, divergence) or None."""
+    s = np.ascontiguousarray(start, np.float64)
+    e = np.ascontiguousarray(end, np.float64)
+    a = np.ascontiguousarray(np.asarray(arcs, np.float64).reshape(-1, 3))
+    along, weight = np.zeros(capacity), np.zeros(capacity)
+    obst = np.zeros(capacity, np.uint8)
+    info = np.zeros(3)
+    n = lib().qmo_line_nodes(_p(s, D), _p(e, D), _p(a, D), a.shape[0], capacity, _p(along, D), _p(weight, D),
+                             _p(obst, U8), _p(info, D))
+    if n < 0:
+        return None
+    return dict(along=along[:n], weight=weight[:n], obstacles=obst[:n].astype(bool), length=info[0],
+                subtended=info[1], divergence=info[2])
+
+
+def skyline_arcs(obstacles: abi.Obstacles, x, y, lo, hi, radius, floor_m=0.0, capacity=1 << 16):
+    out = np.zeros((capacity, 3))
+    n = lib().qmo_skyline_arcs(C.byref(obstacles.c), x, y, lo, hi, radius, floor_m, capacity, _p(out, D))
+    return out[:min(n, capacity)]
+
+
+def evaluate_line_pairs(scene: abi.Scene, points, pieces, pair_point, pair_piece, threads=None):
+    points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+    pieces = np.ascontiguousarray(pieces, abi.LINE_PIECE_DTYPE)
+    pp = np.ascontiguousarray(pair_point, np.uint32)
+    pl = np.ascontiguousarray(pair_piece, np.uint32)
+    n = pp.size
+    sums = np.zeros((n, abi.PAIR_SUMS))
+    failed = np.zeros(n, np.uint32)
+    rays = np.zeros(n, np.uint32)
+    rc = lib().qmo_evaluate_line_pairs(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), C.byref(scene.weather.c),
+                                       points.ctypes.data, points.size, pieces.ctypes.data, pieces.size, _p(pp, U32),
+                                       _p(pl, U32), n, _p(sums, D), _p(failed, U32), _p(rays, U32),
+                                       threads or os.cpu_count() or 1)
+    if rc != 0:
+        raise ValueError("oracle: pair index out of range")
+    return dict(sums=sums, failed=failed, rays=rays)
 
 
 def paint_points(scene: abi.Scene, points, sources, cutoff_m, threads=None):

@@ -48,6 +48,28 @@ struct PairScratch {
     RayTerms terms;
 };
 
+// The line-source quadrature (oracle/line.cpp; physics/src/line.rs).
+struct LineGeometry {
+    double start[3], unit[3];
+    double length_m, foot_along_m, perpendicular_m, start_angle_rad, end_angle_rad;
+};
+struct LineNode {
+    double along_m, weight_rad;
+    bool obstacles_on_ray;
+};
+struct SkylineArc {
+    double lo_rad, hi_rad, nearest_m;
+};
+using SkylineQuery =
+    std::function<void(double lo, double hi, double radius, const std::function<void(const SkylineArc&)>& visit)>;
+double wrap_to_pi(double angle);
+bool line_geometry(const double start[3], const double end[3], LineGeometry& g);
+double line_subtended(const LineGeometry& g);
+double line_divergence_factor(const LineGeometry& g);
+void line_quadrature_nodes(const LineGeometry& g, const SkylineQuery& skyline, std::vector<LineNode>& nodes);
+void skyline_arcs(const qm_obstacles& O, const double origin[2], double lo_rad, double hi_rad, double radius_m,
+                  double los_floor_m, const std::function<void(const SkylineArc&)>& visit);
+
 int ground_at(const qm_terrain& T, double x, double y, GroundSample& out);
 int fill_profile(const qm_terrain& T, const double source[2], const double receiver[2], Profile& profile);
 void steps_of(const qm_obstacles& O, double x, double y, double out[2]);
@@ -62,5 +84,7 @@ int receiver_of(const Scene& S, const qm_point& point, Receiver& out);
 int evaluate_pair(const Scene& S, const Receiver& receiver, const qm_source& source, PairScratch& s, PairResult& out,
                   WorkCount* work);
 void parallel_for(size_t n, int threads, const std::function<void(size_t, int)>& body);
+int evaluate_line_pair(const Scene& S, const Receiver& receiver, const qm_line_piece& piece, PairScratch& s,
+                       PairResult& out, std::vector<LineNode>& nodes, WorkCount* work);
 
 }  // namespace qmo

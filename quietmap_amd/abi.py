@@ -52,6 +52,14 @@ SOURCE_DTYPE = np.dtype(
     ]
 )
 POINT_DTYPE = np.dtype([("x_m", "<f8"), ("y_m", "<f8"), ("own_footprint", "<u8")])
+# qm_line_piece (qm_scene.h): a straight line-source piece and its emission per metre.
+LINE_PIECE_DTYPE = np.dtype([("ax_m", "<f8"), ("ay_m", "<f8"), ("bx_m", "<f8"), ("by_m", "<f8"),
+                             ("height_m", "<f4"), ("layer", "u1"), ("pad", "u1", 3),
+                             ("energy_per_m", "<f4", (3, 8))])
+assert LINE_PIECE_DTYPE.itemsize == 136
+# qm_tile (qm_tiles.h): a tile's receivers (row-major nx x ny from receiver_first) and its sources.
+TILE_DTYPE = np.dtype([("receiver_first", "<u4"), ("nx", "<u4"), ("ny", "<u4"), ("source_first", "<u4"),
+                       ("source_count", "<u4")])
 assert RUN_DTYPE.itemsize == 8 and OUTLINE_DTYPE.itemsize == 24
 assert SOURCE_DTYPE.itemsize == 136 and POINT_DTYPE.itemsize == 24
 

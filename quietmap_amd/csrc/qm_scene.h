@@ -105,6 +105,17 @@ typedef struct qm_source {
     float energy[QM_PERIODS][QM_BANDS]; /* A-weighted sound energy per period and band */
 } qm_source;
 
+/* A straight line-source piece: a line candidate's ends and attribute (popup/src/candidates.rs:
+ * 35-49 with `line` set; physics/src/line.rs integrates it). Ends in frame metres, the source's
+ * height above the ground under it, the emission per metre of the piece. */
+typedef struct qm_line_piece {
+    double ax_m, ay_m, bx_m, by_m;
+    float height_m;
+    uint8_t layer;
+    uint8_t pad[3];
+    float energy_per_m[QM_PERIODS][QM_BANDS]; /* A-weighted sound energy per metre */
+} qm_line_piece;
+
 /* A receiver point (paint/src/exact.rs:18-34): where it stands and the building it stands in. */
 typedef struct qm_point {
     double x_m, y_m;

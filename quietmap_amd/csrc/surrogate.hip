@@ -173,79 +173,83 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
     }
 }
 
-// Inference, all five layers in one workgroup per 64 rows: the activations stay in LDS (two
-// BF16 buffers of 64 x 264), each of the four waves computes 64 output columns of a hidden layer
-// as 2x2 blocks of 32x32 MFMA accumulators with its weights read straight from L2 (16 bytes of
-// one column's K run per lane), and GELU is taken as z * sigmoid(2u), the tanh form's identity.
-constexpr int INFER_ROWS = 64;
+// Inference, all five layers in one workgroup per 128 rows: the activations stay in one LDS
+// buffer (128 x 264 BF16; a layer's products are held in the accumulators until every wave has
+// read its inputs, then the epilogue overwrites them), each of the four waves computes 64 output
+// columns of a hidden layer as 4x2 blocks of 32x32 MFMA accumulators with its weights read from
+// L2 one K step ahead (16 bytes of one column's K run per lane), and GELU is taken as
+// z * sigmoid(2u), the tanh form's identity, with the hardware exp and reciprocal.
+constexpr int INFER_ROWS = 128;
 constexpr int INFER_PITCH = QM_MLP_HIDDEN + 8;
 
 __device__ __forceinline__ float gelu_infer(float z) {
     const float u = GELU_C * (z + GELU_A * z * z * z);
-    return z / (1.0f + __expf(-2.0f * u));
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
-__global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t* __restrict__ x, size_t rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_infer_kernel(const uint16_t* __restrict__ x, size_t rows,
                                                         const uint16_t* __restrict__ wb, const float* __restrict__ w,
                                                         float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint16_t act[2][INFER_ROWS][INFER_PITCH];
+    __shared__ __attribute__((aligned(16))) uint16_t act[INFER_ROWS][INFER_PITCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t row0 = (size_t)blockIdx.x * INFER_ROWS;
     constexpr int K0 = 48;  // the 40 inputs padded to three MFMA K steps
     for (int e = tid; e < INFER_ROWS * K0; e += 256) {
         const int r = e / K0, k = e % K0;
-        act[0][r][k] = (k < QM_MLP_INPUTS && row0 + r < rows) ? x[(row0 + r) * QM_MLP_INPUTS + k] : (uint16_t)0;
+        act[r][k] = (k < QM_MLP_INPUTS && row0 + r < rows) ? x[(row0 + r) * QM_MLP_INPUTS + k] : (uint16_t)0;
     }
     __syncthreads();
-    int cur = 0;
     const int lr = lane & 31, lk = 8 * (lane >> 5);
+    constexpr int RB = INFER_ROWS / 32;
 #pragma unroll 1
     for (int l = 0; l < LAYERS - 1; ++l) {
         const int K = l == 0 ? K0 : QM_MLP_HIDDEN;
         const int fan_in = FAN_IN[l];
-        const uint16_t* W = wb + LAYOUT.w[l];
-        const float* bias = w + LAYOUT.b[l];
         const int c0 = wave * 64;
-        f32x16_t acc[2][2] = {};
-#pragma unroll 4
+        const uint16_t* W0 = wb + LAYOUT.w[l] + (size_t)(c0 + lr) * fan_in + lk;
+        const uint16_t* W1 = W0 + (size_t)32 * fan_in;
+        f32x16_t acc[RB][2] = {};
+        bf16x8_t b0 = *reinterpret_cast<const bf16x8_t*>(W0), b1 = *reinterpret_cast<const bf16x8_t*>(W1);
+#pragma unroll 2
         for (int k0 = 0; k0 < K; k0 += 16) {
-            bf16x8_t a[2], b[2];
+            // The next step's weights are in flight while this step's products run.
+            const int kn = k0 + 16 < K ? k0 + 16 : k0;
+            const bf16x8_t n0 = *reinterpret_cast<const bf16x8_t*>(W0 + kn);
+            const bf16x8_t n1 = *reinterpret_cast<const bf16x8_t*>(W1 + kn);
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-                a[rb] = *reinterpret_cast<const bf16x8_t*>(&act[cur][rb * 32 + lr][k0 + lk]);
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-                b[cb] = *reinterpret_cast<const bf16x8_t*>(W + (size_t)(c0 + cb * 32 + lr) * fan_in + k0 + lk);
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb)
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+            for (int rb = 0; rb < RB; ++rb) {
+                const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[rb * 32 + lr][k0 + lk]);
+                acc[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[rb][0], 0, 0, 0);
+                acc[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[rb][1], 0, 0, 0);
+            }
+            b0 = n0;
+            b1 = n1;
         }
+        __syncthreads();  // every wave has read this layer's inputs
+        const float* bias = w + LAYOUT.b[l];
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
             const int col = c0 + cb * 32 + lr;
             const float bc = bias[col];
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int row = rb * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                    act[cur ^ 1][row][col] = bf16_bits(gelu_infer(acc[rb][cb][r] + bc));
+                    act[row][col] = bf16_bits(gelu_infer(acc[rb][cb][r] + bc));
                 }
         }
         __syncthreads();
-        cur ^= 1;
     }
-    // The 8 outputs: waves 0 and 1 each take 32 rows over one 32-column block (8 used).
-    if (wave < 2) {
+    // The 8 outputs: each wave takes 32 rows over one 32-column block (8 used).
+    {
         const uint16_t* W = wb + LAYOUT.w[LAYERS - 1];
         const float* bias = w + LAYOUT.b[LAYERS - 1];
         f32x16_t acc = {};
         const bf16x8_t zero = {};
 #pragma unroll 4
         for (int k0 = 0; k0 < QM_MLP_HIDDEN; k0 += 16) {
-            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[cur][wave * 32 + lr][k0 + lk]);
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[wave * 32 + lr][k0 + lk]);
             const bf16x8_t b =
                 lr < QM_MLP_OUTPUTS ? *reinterpret_cast<const bf16x8_t*>(W + (size_t)lr * QM_MLP_HIDDEN + k0 + lk) : zero;
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);

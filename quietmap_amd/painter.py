@@ -44,6 +44,11 @@ EXPORTS = (
     "qm_mlp_infer_device",
     "qm_surrogate_samples_device",
     "qm_surrogate_paint_receivers_device",
+    "qm_tiles_plan",
+    "qm_tiles_run",
+    "qm_tiles_paths",
+    "qm_tiles_items",
+    "qm_tiles_plan_free",
 )
 RECEIVER_BYTES = 256
 
@@ -87,6 +92,14 @@ def library():
         lib.qm_mlp_infer_device.argtypes = [vp, vp, sz, vp, vp, cp, sz]
         lib.qm_surrogate_samples_device.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_surrogate_paint_receivers_device.argtypes = [vp, vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, vp, cp, sz]
+        lib.qm_tiles_plan.argtypes = [vp, vp, vp, sz, vp, sz, u32, u32, vp, vp, vp, C.POINTER(vp), cp, sz]
+        lib.qm_tiles_run.argtypes = [vp, vp, cp, sz]
+        lib.qm_tiles_paths.argtypes = [vp]
+        lib.qm_tiles_paths.restype = C.c_uint64
+        lib.qm_tiles_items.argtypes = [vp]
+        lib.qm_tiles_items.restype = C.c_uint64
+        lib.qm_tiles_plan_free.argtypes = [vp]
+        lib.qm_tiles_plan_free.restype = None
         _lib = lib
     return _lib
 

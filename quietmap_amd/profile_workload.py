@@ -3,7 +3,8 @@
     python -m quietmap_amd.profile_workload --kernels paint,loudness,aircraft,copy
 
 `paint` paints the central `--window` x `--window` receivers of the tile-scale scene at `--scale`
-in one paint_kernel launch (the defaults are the benchmark's step), `loudness` runs
+in one paint_kernel launch (the defaults are the benchmark's step), `surrogate` paints the same
+window with the trained network (quietmap_amd/surrogate_params.npy), `loudness` runs
 zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
 `--aircraft-scale`, and `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
@@ -47,6 +48,28 @@ def main():
         hip.synchronize()
         paths = int(d_p.to_array(np.uint32, n).astype(np.int64).sum())
         print(f"paint: {nx}x{ny} receivers, {paths} paths, {time.time() - t:.2f} s", flush=True)
+    if "surrogate" in kernels:
+        import os
+
+        from . import surrogate
+
+        sc = scenes.tile_scene(scale=args.scale)
+        points, nx, ny = sc.window(args.window)
+        n = nx * ny
+        square = gpu.upload(sc.scene, sc.sources)
+        d_points = hip.DeviceBuffer.from_array(points)
+        d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
+        d_e, d_p, d_f, d_s = (hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4),
+                              hip.DeviceBuffer(n * 4))
+        gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)
+        params = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_params.npy"))
+        net = surrogate.Mlp(gpu, params, max_batch=1)
+        t = time.time()
+        surrogate.paint_receivers_device(gpu, square, net, d_rec.ptr, nx, ny, sc.cutoff_m, d_e.ptr, d_p.ptr, d_f.ptr,
+                                         d_s.ptr)
+        hip.synchronize()
+        paths = int(d_p.to_array(np.uint32, n).astype(np.int64).sum())
+        print(f"surrogate: {nx}x{ny} receivers, {paths} paths, {time.time() - t:.2f} s", flush=True)
     if "loudness" in kernels:
         planes = np.random.default_rng(4).uniform(20.0, 90.0, (28, args.cells)).astype(np.float32)
         d_levels = hip.DeviceBuffer.from_array(planes)

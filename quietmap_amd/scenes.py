@@ -365,3 +365,86 @@ def aircraft_scene(seed=3, scale=1.0, receivers=2048, spacing=50.0, classes=100,
     jj, ii = np.mgrid[0:n, 0:n]
     xy = np.stack([(ii.reshape(-1) + 0.5) * spacing, (jj.reshape(-1) + 0.5) * spacing], 1).astype(np.float64)
     return AircraftScene(np.ascontiguousarray(xy), (n, n), cl, tr, verts.reshape(-1, 3), 4.0, reach_m)
+
+
+class TileMosaic:
+    """Tiles of receivers, each painted against its own sources only (the batched configuration).
+
+    `scene` covers the whole mosaic; `points` are tile-major (tile t's receivers are
+    points[t*nx*ny : (t+1)*nx*ny], row-major); `sources` are sorted by tile, tile t's being
+    sources[source_first[t] : source_first[t+1]]."""
+
+    def __init__(self, scene, sources, points, tile_shape, tiles_xy, source_first):
+        self.scene, self.sources, self.points = scene, sources, points
+        self.tile_shape, self.tiles_xy, self.source_first = tile_shape, tiles_xy, source_first
+
+    @property
+    def n_tiles(self):
+        return self.tiles_xy[0] * self.tiles_xy[1]
+
+    def tile_table(self):
+        """The qm_tile records of every tile (abi.TILE_DTYPE)."""
+        nx, ny = self.tile_shape
+        t = np.zeros(self.n_tiles, abi.TILE_DTYPE)
+        t["receiver_first"] = np.arange(self.n_tiles, dtype=np.uint32) * nx * ny
+        t["nx"], t["ny"] = nx, ny
+        t["source_first"] = self.source_first[:-1]
+        t["source_count"] = np.diff(self.source_first)
+        return t
+
+
+def tile_mosaic(seed=4, tiles=(4, 4), receivers=256, spacing=10.0, source_median=500, building_median=300,
+                sigma=1.0, source_clip=(10, 50_000), building_clip=(1, 50_000)):
+    """Configuration "batched uneven tiles": tiles of receivers x receivers at `spacing`, per tile a
+    log-normal count of point sources (median 500, sigma 1, clipped to 10..50k) and of buildings
+    (median 300), over one diamond-square DEM (30 m lattice, 200 m relief)."""
+    rng = np.random.default_rng(seed)
+    tx, ty = tiles
+    side = receivers * spacing
+    extent = (tx * side, ty * side)
+    margin = 600.0
+    nodes_needed = int(math.ceil((max(extent) + 2 * margin) / 30.0)) + 1
+    power = max(int(math.ceil(math.log2(max(nodes_needed - 1, 2)))), 2)
+    dem = _diamond_square(rng, power, 200.0)[:nodes_needed, :nodes_needed] + 100.0
+    impervious = np.clip(rng.normal(40, 30, dem.shape), 0, 100).astype(np.uint32)
+    terrain = abi.Terrain(-margin, -margin, 30.0, abi.Terrain.pack(dem, impervious))
+    n_tiles = tx * ty
+    n_src = np.clip(np.round(source_median * np.exp(sigma * rng.normal(size=n_tiles))), *source_clip).astype(int)
+    n_bld = np.clip(np.round(building_median * np.exp(sigma * rng.normal(size=n_tiles))), *building_clip).astype(int)
+    rings, heights = [], []
+    all_src, first = [], [0]
+    for t in range(n_tiles):
+        ox, oy = (t % tx) * side, (t // tx) * side
+        for cx, cy in rng.uniform(0, side, (n_bld[t], 2)) + (ox, oy):
+            k = int(rng.integers(4, 13))
+            ang = np.sort(rng.uniform(0, 2 * math.pi, k))
+            rad = rng.uniform(5.0, 20.0) * rng.uniform(0.6, 1.0, k)
+            rings.append(list(zip(cx + rad * np.cos(ang), cy + rad * np.sin(ang))))
+            heights.append(rng.uniform(3.0, 60.0))
+        per_line = 10
+        xy = _polyline_points(rng, int(math.ceil(n_src[t] / per_line)), per_line, (side, side), (ox, oy))[:n_src[t]]
+        src = np.zeros(len(xy), abi.SOURCE_DTYPE)
+        src["x_m"], src["y_m"] = xy[:, 0], xy[:, 1]
+        rail = rng.uniform(0, 1, len(xy)) < 0.3
+        src["height_m"] = np.where(rail, 0.5, 0.05)
+        src["layer"] = np.where(rail, 1, 0)
+        src["exclusion_radius_m"] = 2.0
+        src["ground_percent"] = abi.GROUND_FROM_TERRAIN
+        levels = rng.normal(85.0, 8.0, (len(xy), 8)) + A_WEIGHTING_DB
+        period_offset = np.array([0.0, -2.0, -7.0])
+        src["energy"] = 10.0 ** ((levels[:, None, :] + period_offset[None, :, None]) / 10.0)
+        all_src.append(src)
+        first.append(first[-1] + len(src))
+    footprints = np.arange(1, len(rings) + 1, dtype=np.uint64)
+    obstacles = _obstacles(rings, heights, [abi.KIND_BUILDING] * len(rings), footprints, (0.0, 0.0), extent)
+    weather = _weather((-margin, -margin), (extent[0] + 2 * margin, extent[1] + 2 * margin),
+                       lambda r, c: rng.integers(20, 80, (3, 16)))
+    pts = []
+    for t in range(n_tiles):
+        ox, oy = (t % tx) * side, (t // tx) * side
+        p = receiver_grid(receivers, receivers, spacing, (ox + spacing / 2, oy + spacing / 2))
+        _own_footprints(p, rings, footprints, spacing, (ox + spacing / 2, oy + spacing / 2), (receivers, receivers))
+        pts.append(p)
+    scene = abi.Scene(terrain, obstacles, weather)
+    return TileMosaic(scene, np.concatenate(all_src), np.concatenate(pts), (receivers, receivers), (tx, ty),
+                      np.array(first, np.uint32))

@@ -57,18 +57,20 @@ def features_reference(sc, point, altitude, source):
     f[0] = H / 1000
     f[1] = source["height_m"] / 4
     f[2] = (altitude - r1[0]) / 4
-    f[3] = (altitude - source_alt) / 100
     forest = s0[2] > 0 or r1[2] > 0
     g_sum = np.zeros(4)
-    for j in range(24):
-        t = (j + 0.5) / 24
+    highest = -np.inf
+    for j in range(48):
+        t = (j + 0.5) / 48
         _, g = oracle.ground_at(T, sx + (rx - sx) * t, sy + (ry - sy) * t)
         above = (g[0] - (source_alt + slope * t * H)) / 10
-        k = 4 + j // 2
-        f[k] = max(f[k], above) if j % 2 else above
-        g_sum[j // 6] += g[1]
+        k = 4 + j // 4
+        f[k] = max(f[k], above) if j % 4 else above
+        if above > highest:
+            highest, f[3] = above, t
+        g_sum[j // 12] += g[1]
         forest = forest or g[2] > 0
-    f[16:20] = g_sum / 6
+    f[16:20] = g_sum / 12
     tops, roofed, open_fp = [], 0.0, None
     for c in cr:
         x = c["t"] * H

@@ -1,5 +1,5 @@
 """The C ABI library loads without a GPU and exports every entry point include/qm_paint.h
-declares (qm_paint.h, qm_surrogate.h); the product path refuses to run without it (no CPU fallback)."""
+declares (qm_paint.h, qm_surrogate.h, qm_tiles.h); the product path refuses to run without it (no CPU fallback)."""
 import ctypes
 import os
 import re
@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     names = set()
-    for header in ("qm_paint.h", "qm_surrogate.h"):
+    for header in ("qm_paint.h", "qm_surrogate.h", "qm_tiles.h"):
         text = open(os.path.join(ROOT, "include", header)).read()
         names |= set(re.findall(r"\b(qm_[a-z_]+)\s*\(", text))
     return sorted(names)
