@@ -101,6 +101,16 @@ int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_cla
                      uint32_t ny, double receiver_altitude_m, double reach_m, double* period_energy, uint32_t* n65,
                      uint32_t* n70, uint32_t* hist, char* err, size_t errlen);
 
+/* Gpu::evaluate_pairs for line candidates (paint-gpu/kernels/paint.cu:77-290 pair_rays, the
+ * count / scan / list of rays, and ray_transfers; popup/src/evaluate.rs:171-267 source_rays): per
+ * pair (point, line piece) the piece's quadrature nodes seen from the point (physics/src/line.rs,
+ * the skyline mask of wide buckets), one exact ray per node, and the pair's 9 sums, failure bits,
+ * rays (optional, per pair) and point bands (optional, as qm_evaluate_pairs). Host arrays. */
+int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
+                           const qm_line_piece* pieces, size_t n_pieces, const uint32_t* pair_point,
+                           const uint32_t* pair_piece, size_t n_pairs, float* sums, uint32_t* failed, uint32_t* rays,
+                           float* point_bands, char* err, size_t errlen);
+
 /* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
  * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings
  * (t, footprint, outline, edge index, flags bit 0 building / bit 1 bounds roof) and the count. */

@@ -45,46 +45,11 @@ struct GridArgs {
 
 }  // namespace
 
-// paint/src/exact.rs:95-118 Point::with_receiver_properties and physics/src/weather.rs:163-214.
+// paint/src/exact.rs:95-118 Point::with_receiver_properties (receiver_of in qm_eval.hpp).
 __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point* points, uint32_t n,
                                                         qm_receiver* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const qm_point p = points[i];
-    qm_receiver r;
-    r.x_m = p.x_m;
-    r.y_m = p.y_m;
-    r.own_footprint = p.own_footprint;
-    const qm_terrain& T = sc.terrain;
-    GroundSample g;
-    const double inv = 1.0 / T.spacing_m;
-    r.failed = ground_at(T, (p.x_m - T.origin_x_m) * inv, (p.y_m - T.origin_y_m) * inv, g);
-    r.altitude_m = r.failed ? 0.0f : g.z + (float)QM_RECEIVER_HEIGHT_M;
-    const qm_weather& W = sc.weather;
-    const double yy = clampd((p.y_m - W.origin_y_m) / W.spacing_m, 0.0, (double)(W.rows - 1));
-    const double xx = clampd((p.x_m - W.origin_x_m) / W.spacing_m, 0.0, (double)(W.columns - 1));
-    const uint32_t row = min((uint32_t)floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
-    const uint32_t col = min((uint32_t)floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
-    const float fy = (float)(yy - row), fx = (float)(xx - col);
-    const uint32_t r1 = W.rows >= 2 ? row + 1 : row, c1 = W.columns >= 2 ? col + 1 : col;
-    const uint8_t* nd[4] = {W.nodes + ((size_t)row * W.columns + col) * QM_WEATHER_NODE_BYTES,
-                            W.nodes + ((size_t)row * W.columns + c1) * QM_WEATHER_NODE_BYTES,
-                            W.nodes + ((size_t)r1 * W.columns + col) * QM_WEATHER_NODE_BYTES,
-                            W.nodes + ((size_t)r1 * W.columns + c1) * QM_WEATHER_NODE_BYTES};
-    const float w[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
-    for (int q = 0; q < PERIODS; ++q)
-        for (int s = 0; s < SECTORS; ++s) {
-            float total = 0.0f;
-            for (int k = 0; k < 4; ++k) total += w[k] * (float)nd[k][q * SECTORS + s];
-            r.favourable[q][s] = total * 0.01f;
-        }
-    for (int b = 0; b < BANDS; ++b) {
-        float total = 0.0f;
-        const int at = PERIODS * SECTORS + 2 * b;
-        for (int k = 0; k < 4; ++k) total += w[k] * (float)((uint32_t)nd[k][at] | ((uint32_t)nd[k][at + 1] << 8));
-        r.alpha_db_per_km[b] = total * 0.01f;
-    }
-    out[i] = r;
+    if (i < n) out[i] = receiver_of(sc, points[i]);
 }
 
 #ifndef QM_PAINT_WAVES_PER_EU

@@ -209,8 +209,55 @@ QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, 
     return 0;
 }
 
+// paint/src/exact.rs:95-118 Point::with_receiver_properties and physics/src/weather.rs:163-214.
+QM_DEV qm_receiver receiver_of(const Scene& sc, const qm_point& p) {
+    qm_receiver r;
+    r.x_m = p.x_m;
+    r.y_m = p.y_m;
+    r.own_footprint = p.own_footprint;
+    const qm_terrain& T = sc.terrain;
+    GroundSample g;
+    const double inv = 1.0 / T.spacing_m;
+    r.failed = ground_at(T, (p.x_m - T.origin_x_m) * inv, (p.y_m - T.origin_y_m) * inv, g);
+    r.altitude_m = r.failed ? 0.0f : g.z + (float)QM_RECEIVER_HEIGHT_M;
+    const qm_weather& W = sc.weather;
+    const double yy = clampd((p.y_m - W.origin_y_m) / W.spacing_m, 0.0, (double)(W.rows - 1));
+    const double xx = clampd((p.x_m - W.origin_x_m) / W.spacing_m, 0.0, (double)(W.columns - 1));
+    const uint32_t row = min((uint32_t)floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
+    const uint32_t col = min((uint32_t)floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
+    const float fy = (float)(yy - row), fx = (float)(xx - col);
+    const uint32_t r1 = W.rows >= 2 ? row + 1 : row, c1 = W.columns >= 2 ? col + 1 : col;
+    const uint8_t* nd[4] = {W.nodes + ((size_t)row * W.columns + col) * QM_WEATHER_NODE_BYTES,
+                            W.nodes + ((size_t)row * W.columns + c1) * QM_WEATHER_NODE_BYTES,
+                            W.nodes + ((size_t)r1 * W.columns + col) * QM_WEATHER_NODE_BYTES,
+                            W.nodes + ((size_t)r1 * W.columns + c1) * QM_WEATHER_NODE_BYTES};
+    const float w[4] = {(1.0f - fy) * (1.0f - fx), (1.0f - fy) * fx, fy * (1.0f - fx), fy * fx};
+    for (int q = 0; q < PERIODS; ++q)
+        for (int s = 0; s < SECTORS; ++s) {
+            float total = 0.0f;
+            for (int k = 0; k < 4; ++k) total += w[k] * (float)nd[k][q * SECTORS + s];
+            r.favourable[q][s] = total * 0.01f;
+        }
+    for (int b = 0; b < BANDS; ++b) {
+        float total = 0.0f;
+        const int at = PERIODS * SECTORS + 2 * b;
+        for (int k = 0; k < 4; ++k) total += w[k] * (float)((uint32_t)nd[k][at] | ((uint32_t)nd[k][at + 1] << 8));
+        r.alpha_db_per_km[b] = total * 0.01f;
+    }
+    return r;
+}
+
+// How a ray is taken: whether walls are searched on it (a line node inside a clear gap of its
+// skyline mask has none: popup/src/evaluate.rs:300-314) and its weight (negative: the point
+// source's divergence, evaluate.rs:195-203; a line node's dphi times the piece's divergence
+// factor otherwise, evaluate.rs:245-265).
+struct RayOptions {
+    bool obstacles = true;
+    float weight = -1.0f;
+};
+
 QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver& rcv, const qm_source& src,
-                          PathScratch& w, PathResult& out) {
+                          PathScratch& w, PathResult& out, const RayOptions& opt = RayOptions{}) {
     uint32_t failed = rcv.failed;
     for (int p = 0; p < PERIODS; ++p) {
         for (int b = 0; b < BANDS; ++b) out.bands[p][b] = 0.0f;
@@ -229,7 +276,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float H = (float)fmax(length_d, 1.0);
 
     int n = 0;
-    failed |= path_crossings(sc, rcv, src, w.cross, n);
+    if (opt.obstacles) failed |= path_crossings(sc, rcv, src, w.cross, n);
     if (failed & QM_FAILED_NOT_READ) {
         out.failed = failed;
         return;
@@ -476,7 +523,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float hd = fmaxf((float)length_d, src.exclusion_radius_m);
     const float dz = receiver_alt - (s0.z + src.height_m);
     const float d = fmaxf(sqrtf(hd * hd + dz * dz), 1.0f);
-    const float weight = 1.0f / (d * d * 12.589254117941673f);
+    const float weight = opt.weight >= 0.0f ? opt.weight : 1.0f / (d * d * 12.589254117941673f);
     float fol_db[2][BANDS];
     for (int s = 0; s < 2; ++s) {
         const float depth = fol.depth[s];

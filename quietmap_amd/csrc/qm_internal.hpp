@@ -34,7 +34,8 @@ struct qm_square {
 
 struct qm_mlp;
 namespace qm {
-// The surrogate network's outputs (rows x 8, FP32) of BF16 input rows (rows x 40) on `stream`.
-int mlp_infer_rows(qm_mlp* mlp, const uint16_t* d_rows, size_t rows, float* d_outputs, hipStream_t stream, char* err,
-                   size_t errlen);
+// The surrogate network's outputs (rows x 8, FP32) of BF16 input rows (rows x 40) on `stream`;
+// with `d_rows_pairs`, the rows are 2 * *d_rows_pairs (at most `rows`), read on the device.
+int mlp_infer_rows(qm_mlp* mlp, const uint16_t* d_rows, size_t rows, const uint32_t* d_rows_pairs, float* d_outputs,
+                   hipStream_t stream, char* err, size_t errlen);
 }  // namespace qm

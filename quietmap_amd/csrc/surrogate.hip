@@ -175,92 +175,134 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
 
 // Inference, all five layers in one workgroup per 128 rows: the activations stay in one LDS
 // buffer (128 x 264 BF16; a layer's products are held in the accumulators until every wave has
-// read its inputs, then the epilogue overwrites them), each of the four waves computes 64 output
-// columns of a hidden layer as 4x2 blocks of 32x32 MFMA accumulators with its weights read from
-// L2 one K step ahead (16 bytes of one column's K run per lane), and GELU is taken as
-// z * sigmoid(2u), the tanh form's identity, with the hardware exp and reciprocal.
-constexpr int INFER_ROWS = 128;
+// read its inputs, then the epilogue overwrites them). Each of the four waves computes 64 output
+// features of a hidden layer for all rows as D = W_block * act_block^T (2 feature blocks x 4 row
+// blocks of 32x32 MFMA accumulators): a lane then holds one row and runs of four consecutive
+// features, which the epilogue converts in pairs (v_cvt_pk_bf16_f32) and stores 8 bytes at a time.
+// Weights are read from L2 a chunk of four K steps ahead; GELU is z * sigmoid(2u), the tanh
+// form's identity, with the hardware exp and reciprocal.
+#ifndef QM_INFER_ROWS
+#define QM_INFER_ROWS 128
+#endif
+#ifndef QM_INFER_WAVES_PER_EU
+#define QM_INFER_WAVES_PER_EU 2
+#endif
+constexpr int INFER_ROWS = QM_INFER_ROWS;
 constexpr int INFER_PITCH = QM_MLP_HIDDEN + 8;
+constexpr int RB = INFER_ROWS / 32;
+
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float gelu_infer(float z) {
     const float u = GELU_C * (z + GELU_A * z * z * z);
     return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_infer_kernel(const uint16_t* __restrict__ x, size_t rows,
-                                                        const uint16_t* __restrict__ wb, const float* __restrict__ w,
-                                                        float* __restrict__ out) {
+// One hidden layer's products for a wave: features f0 .. f0+63 (two 32-feature blocks) over all
+// INFER_ROWS rows, K in chunks of four 16-wide steps, the next chunk's weights loaded while this
+// chunk's products run (the scheduling barriers keep the loads a whole chunk ahead).
+template <int K>
+__device__ __forceinline__ void hidden_products(const uint16_t (*act)[INFER_PITCH], const uint16_t* W0,
+                                                const uint16_t* W1, int lr, int lk, f32x16_t (&acc)[2][RB]) {
+    constexpr int STEPS = K / 16;
+    constexpr int CHUNK = STEPS < 4 ? STEPS : 4;
+    constexpr int CHUNKS = STEPS / CHUNK;
+    static_assert(STEPS % CHUNK == 0, "K steps per chunk");
+    bf16x8_t w[2][CHUNK][2];
+#pragma unroll
+    for (int i = 0; i < CHUNK; ++i) {
+        w[0][i][0] = *reinterpret_cast<const bf16x8_t*>(W0 + 16 * i);
+        w[0][i][1] = *reinterpret_cast<const bf16x8_t*>(W1 + 16 * i);
+    }
+#pragma unroll
+    for (int c = 0; c < CHUNKS; ++c) {
+        if (c + 1 < CHUNKS) {
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                w[(c + 1) & 1][i][0] = *reinterpret_cast<const bf16x8_t*>(W0 + 16 * ((c + 1) * CHUNK + i));
+                w[(c + 1) & 1][i][1] = *reinterpret_cast<const bf16x8_t*>(W1 + 16 * ((c + 1) * CHUNK + i));
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) {
+            const int st = c * CHUNK + i;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const bf16x8_t x = *reinterpret_cast<const bf16x8_t*>(&act[rb * 32 + lr][16 * st + lk]);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][0], x, acc[0][rb], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][1], x, acc[1][rb], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// `rows` rows, or 2 * *rows_pairs when that count is given (the rows a device kernel produced).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_INFER_WAVES_PER_EU))) void mlp_infer_kernel(
+    const uint16_t* __restrict__ x, size_t rows, const uint32_t* __restrict__ rows_pairs,
+    const uint16_t* __restrict__ wb, const float* __restrict__ w, float* __restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint16_t act[INFER_ROWS][INFER_PITCH];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (rows_pairs) rows = 2 * (size_t)*rows_pairs;
     const size_t row0 = (size_t)blockIdx.x * INFER_ROWS;
+    if (row0 >= rows) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int K0 = 48;  // the 40 inputs padded to three MFMA K steps
     for (int e = tid; e < INFER_ROWS * K0; e += 256) {
         const int r = e / K0, k = e % K0;
         act[r][k] = (k < QM_MLP_INPUTS && row0 + r < rows) ? x[(row0 + r) * QM_MLP_INPUTS + k] : (uint16_t)0;
     }
     __syncthreads();
-    const int lr = lane & 31, lk = 8 * (lane >> 5);
-    constexpr int RB = INFER_ROWS / 32;
+    const int lr = lane & 31, lk = 8 * (lane >> 5), lq = 4 * (lane >> 5);
 #pragma unroll 1
     for (int l = 0; l < LAYERS - 1; ++l) {
-        const int K = l == 0 ? K0 : QM_MLP_HIDDEN;
         const int fan_in = FAN_IN[l];
-        const int c0 = wave * 64;
-        const uint16_t* W0 = wb + LAYOUT.w[l] + (size_t)(c0 + lr) * fan_in + lk;
+        const int f0 = wave * 64;
+        const uint16_t* W0 = wb + LAYOUT.w[l] + (size_t)(f0 + lr) * fan_in + lk;
         const uint16_t* W1 = W0 + (size_t)32 * fan_in;
-        f32x16_t acc[RB][2] = {};
-        bf16x8_t b0 = *reinterpret_cast<const bf16x8_t*>(W0), b1 = *reinterpret_cast<const bf16x8_t*>(W1);
-#pragma unroll 2
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            // The next step's weights are in flight while this step's products run.
-            const int kn = k0 + 16 < K ? k0 + 16 : k0;
-            const bf16x8_t n0 = *reinterpret_cast<const bf16x8_t*>(W0 + kn);
-            const bf16x8_t n1 = *reinterpret_cast<const bf16x8_t*>(W1 + kn);
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[rb * 32 + lr][k0 + lk]);
-                acc[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[rb][0], 0, 0, 0);
-                acc[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[rb][1], 0, 0, 0);
-            }
-            b0 = n0;
-            b1 = n1;
-        }
+        f32x16_t acc[2][RB] = {};
+        if (l == 0) hidden_products<K0>(act, W0, W1, lr, lk, acc);
+        else hidden_products<QM_MLP_HIDDEN>(act, W0, W1, lr, lk, acc);
         __syncthreads();  // every wave has read this layer's inputs
         const float* bias = w + LAYOUT.b[l];
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-            const int col = c0 + cb * 32 + lr;
-            const float bc = bias[col];
+        for (int fb = 0; fb < 2; ++fb)
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
+            for (int g = 0; g < 4; ++g) {
+                // Features f0 + 32 fb + 8 g + lq .. +3 of row (rb, lr).
+                const int feature = f0 + fb * 32 + 8 * g + lq;
+                const float4 bc = *reinterpret_cast<const float4*>(bias + feature);
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int row = rb * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                    act[row][col] = bf16_bits(gelu_infer(acc[rb][cb][r] + bc));
+                for (int rb = 0; rb < RB; ++rb) {
+                    bf16x4_t v;
+                    v.x = (__bf16)gelu_infer(acc[fb][rb][4 * g + 0] + bc.x);
+                    v.y = (__bf16)gelu_infer(acc[fb][rb][4 * g + 1] + bc.y);
+                    v.z = (__bf16)gelu_infer(acc[fb][rb][4 * g + 2] + bc.z);
+                    v.w = (__bf16)gelu_infer(acc[fb][rb][4 * g + 3] + bc.w);
+                    *reinterpret_cast<bf16x4_t*>(&act[rb * 32 + lr][feature]) = v;
                 }
-        }
+            }
         __syncthreads();
     }
-    // The 8 outputs: each wave takes 32 rows over one 32-column block (8 used).
-    {
+    // The 8 outputs: 32-row blocks dealt over the waves, D = W5 (8 of 32 feature rows) * act^T.
+    for (int rb = wave; rb < RB; rb += 4) {
         const uint16_t* W = wb + LAYOUT.w[LAYERS - 1];
         const float* bias = w + LAYOUT.b[LAYERS - 1];
         f32x16_t acc = {};
         const bf16x8_t zero = {};
 #pragma unroll 4
         for (int k0 = 0; k0 < QM_MLP_HIDDEN; k0 += 16) {
-            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&act[wave * 32 + lr][k0 + lk]);
-            const bf16x8_t b =
+            const bf16x8_t xa = *reinterpret_cast<const bf16x8_t*>(&act[rb * 32 + lr][k0 + lk]);
+            const bf16x8_t wa =
                 lr < QM_MLP_OUTPUTS ? *reinterpret_cast<const bf16x8_t*>(W + (size_t)lr * QM_MLP_HIDDEN + k0 + lk) : zero;
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, xa, acc, 0, 0, 0);
         }
-        if (lr < QM_MLP_OUTPUTS) {
-            const float bc = bias[lr];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const size_t row = row0 + wave * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                if (row < rows) out[row * QM_MLP_OUTPUTS + lr] = acc[r] + bc;
-            }
+        // acc[0..3]: features lq .. lq+3 of row (rb, lr).
+        const size_t row = row0 + rb * 32 + lr;
+        if (row < rows) {
+            const float4 bc = *reinterpret_cast<const float4*>(bias + lq);
+            *reinterpret_cast<float4*>(out + row * QM_MLP_OUTPUTS + lq) =
+                make_float4(acc[0] + bc.x, acc[1] + bc.y, acc[2] + bc.z, acc[3] + bc.w);
         }
     }
 }
@@ -589,12 +631,13 @@ extern "C" int qm_mlp_train_step_device(qm_mlp* p, const float* d_inputs, const 
     return 0;
 }
 
-int qm::mlp_infer_rows(qm_mlp* p, const uint16_t* d_rows, size_t rows, float* d_outputs, hipStream_t s, char* err,
-                       size_t errlen) {
+int qm::mlp_infer_rows(qm_mlp* p, const uint16_t* d_rows, size_t rows, const uint32_t* d_rows_pairs, float* d_outputs,
+                       hipStream_t s, char* err, size_t errlen) {
     if (rows == 0) return 0;
     const size_t blocks = (rows + INFER_ROWS - 1) / INFER_ROWS;
     if (blocks > 0x7fffffffu) return fail(err, errlen, "too many rows");
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_rows, rows, p->wb, p->w, d_outputs);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_rows, rows, d_rows_pairs, p->wb,
+                       p->w, d_outputs);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(err, errlen, "mlp_infer_kernel: %s", hipGetErrorString(e));
     return 0;
@@ -613,5 +656,5 @@ extern "C" int qm_mlp_infer_device(qm_mlp* p, const float* d_inputs, size_t rows
         e = hipGetLastError();
     }
     if (e != hipSuccess) return fail(err, errlen, "qm_mlp_infer_device: %s", hipGetErrorString(e));
-    return qm::mlp_infer_rows(p, p->xb, rows, d_outputs, s, err, errlen);
+    return qm::mlp_infer_rows(p, p->xb, rows, nullptr, d_outputs, s, err, errlen);
 }

@@ -151,8 +151,9 @@ __device__ __forceinline__ bool tile_receiver(const TileGrid& g, uint32_t tile, 
 // Pair flags: 0 no path (outside the grid or the cutoff), 1 the network's, 2 the exact path's.
 __global__ __launch_bounds__(256) void features_kernel(Scene sc, TileGrid g, const qm_source* sources,
                                                        const uint32_t* item_tile, const uint32_t* item_source,
-                                                       uint16_t* rows, uint8_t* flags, float* source_alt,
-                                                       uint32_t* exact_list, uint32_t* exact_count) {
+                                                       uint16_t* rows, uint32_t* row_count, uint32_t* row_of,
+                                                       uint8_t* flags, float* source_alt, uint32_t* exact_list,
+                                                       uint32_t* exact_count) {
     const uint32_t item = blockIdx.x;
     const size_t pair = (size_t)item * 256 + threadIdx.x;
     uint32_t index;
@@ -160,21 +161,23 @@ __global__ __launch_bounds__(256) void features_kernel(Scene sc, TileGrid g, con
     const qm_receiver& rcv = g.receivers[index];
     const qm_source& src = sources[item_source[item]];
     const double dx = src.x_m - rcv.x_m, dy = src.y_m - rcv.y_m;
-    uint16_t* x0 = rows + pair * 2 * FEATURES;
     if (!valid || dx * dx + dy * dy > g.cutoff * g.cutoff) {
         flags[pair] = 0;
-        for (int k = 0; k < 2 * FEATURES; ++k) x0[k] = 0;
         return;
     }
     Crossing cross[CROSS_CAP];
     float f[FEATURES];
     const SurrogatePath sp = surrogate_features(sc, rcv, src, cross, f);
-    for (int k = 0; k < FEATURES; ++k) {
-        const uint16_t h = to_bf16(f[k]);
-        x0[k] = h;
-        x0[FEATURES + k] = k == 37 ? to_bf16(1.0f) : h;
-    }
     if (sp.in_envelope) {
+        // The network's rows of this pair: the next free pair of rows (calm, favourable).
+        const uint32_t slot = atomicAdd(row_count, 1u);
+        row_of[pair] = slot;
+        uint16_t* x0 = rows + (size_t)slot * 2 * FEATURES;
+        for (int k = 0; k < FEATURES; ++k) {
+            const uint16_t h = to_bf16(f[k]);
+            x0[k] = h;
+            x0[FEATURES + k] = k == 37 ? to_bf16(1.0f) : h;
+        }
         flags[pair] = 1;
         // The source's altitude for the energy terms (its ground as the exact path samples it).
         const qm_terrain& T = sc.terrain;
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void combine_kernel(TileGrid g, uint32_t first
                                                       const uint32_t* tile_items, uint32_t chunk_item0,
                                                       const uint32_t* block_tile, const uint32_t* block_item0,
                                                       const uint32_t* item_source, const uint8_t* flags,
-                                                      const float* source_alt, const float* att,
+                                                      const uint32_t* row_of, const float* source_alt, const float* att,
                                                       const float* exact_energy, const uint8_t* exact_failed,
                                                       Partial* partials) {
     const uint32_t blk = blockIdx.x;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(256) void combine_kernel(TileGrid g, uint32_t first
             const float hd = fmaxf((float)length_d, src.exclusion_radius_m);
             const float d = fmaxf(sqrtf(hd * hd + dz * dz), 1.0f);
             const float weight = 1.0f / (d * d * 12.589254117941673f);
-            const float* a0 = att + p * 2 * BANDS;
+            const float* a0 = att + (size_t)row_of[p] * 2 * BANDS;
             float bands[PERIODS][BANDS];
             for (int b = 0; b < BANDS; ++b) {
                 const float air = rcv.alpha_db_per_km[b] * slant * 1e-3f;
@@ -408,13 +411,16 @@ extern "C" int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square*
     // Chunks of whole tiles with at most CHUNK_ITEMS items.
     const size_t I = CHUNK_ITEMS, P = CHUNK_ITEMS * 256;
     const size_t max_blocks = I / COMBINE_ITEMS + (size_t)n_tiles;
-    uint32_t *d_exact_list, *d_exact_count, *d_chunk, *d_block_tile, *d_block_item0, *d_tile_blocks;
+    uint32_t *d_exact_list, *d_exact_count, *d_row_count, *d_row_of, *d_chunk, *d_block_tile, *d_block_item0,
+        *d_tile_blocks;
     uint16_t* d_rows;
     uint8_t *d_flags, *d_exact_failed;
     float *d_source_alt, *d_att, *d_exact_energy;
     Partial* d_partials;
     SP_TRY(mem.alloc((void**)&d_exact_list, P * 4));
     SP_TRY(mem.alloc((void**)&d_exact_count, 4));
+    SP_TRY(mem.alloc((void**)&d_row_count, 4));
+    SP_TRY(mem.alloc((void**)&d_row_of, P * 4));
     SP_TRY(mem.alloc((void**)&d_rows, P * 2 * FEATURES * 2));
     SP_TRY(mem.alloc((void**)&d_flags, P));
     SP_TRY(mem.alloc((void**)&d_exact_failed, P));
@@ -459,12 +465,13 @@ extern "C" int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square*
         d_block_item0 = d_block_tile + B;
         d_tile_blocks = d_block_item0 + B;
         hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, s, d_exact_count);
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, s, d_row_count);
         if (items) {
             hipLaunchKernelGGL(features_kernel, dim3((unsigned)items), dim3(256), 0, s, square->scene, g,
-                               square->grid.sources, d_item_tile + item0, d_item_source + item0, d_rows, d_flags,
-                               d_source_alt, d_exact_list, d_exact_count);
+                               square->grid.sources, d_item_tile + item0, d_item_source + item0, d_rows, d_row_count,
+                               d_row_of, d_flags, d_source_alt, d_exact_list, d_exact_count);
             SP_TRY(hipGetLastError());
-            if (qm::mlp_infer_rows(mlp, d_rows, items * 256 * 2, d_att, s, err, errlen)) return 1;
+            if (qm::mlp_infer_rows(mlp, d_rows, items * 256 * 2, d_row_count, d_att, s, err, errlen)) return 1;
             hipLaunchKernelGGL(exact_kernel, dim3((unsigned)std::max(cus, 1) * 4), dim3(256), 0, s, square->scene,
                                gpu->bands, g, square->grid.sources, d_item_tile + item0, d_item_source + item0,
                                d_exact_list, d_exact_count, d_exact_energy, d_exact_failed);
@@ -472,8 +479,8 @@ extern "C" int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square*
         }
         if (B)
             hipLaunchKernelGGL(combine_kernel, dim3(B), dim3(256), 0, s, g, first, square->grid.sources, d_tile_items,
-                               item0, d_block_tile, d_block_item0, d_item_source, d_flags, d_source_alt, d_att,
-                               d_exact_energy, d_exact_failed, d_partials);
+                               item0, d_block_tile, d_block_item0, d_item_source, d_flags, d_row_of, d_source_alt,
+                               d_att, d_exact_energy, d_exact_failed, d_partials);
         hipLaunchKernelGGL(reduce_kernel, dim3(m), dim3(256), 0, s, g, first, d_tile_blocks, d_partials,
                            d_period_energy, d_paths, d_failures, d_surrogate_paths);
         SP_TRY(hipGetLastError());

@@ -36,6 +36,7 @@ EXPORTS = (
     "qm_loudness_device",
     "qm_aircraft_grid",
     "qm_crossings",
+    "qm_evaluate_line_pairs",
     "qm_mlp_new",
     "qm_mlp_params",
     "qm_mlp_free",
@@ -82,6 +83,7 @@ def library():
         lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
+        lib.qm_evaluate_line_pairs.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, cp, sz]
         lib.qm_mlp_new.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
         lib.qm_mlp_params.argtypes = [vp, vp, cp, sz]
@@ -247,6 +249,24 @@ class Gpu:
                                         scene.reach_m, _ptr(energy), _ptr(n65), _ptr(n70), _ptr(hist), err, 512)
         _check(rc, err)
         return dict(period_energy=energy, n65=n65, n70=n70, hist=hist)
+
+    def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False):
+        """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands."""
+        pts = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        pcs = np.ascontiguousarray(pieces, abi.LINE_PIECE_DTYPE)
+        pp = np.ascontiguousarray(pair_point, np.uint32)
+        pl = np.ascontiguousarray(pair_piece, np.uint32)
+        n = pp.size
+        sums = np.zeros((n, abi.PAIR_SUMS), np.float32)
+        failed = np.zeros(n, np.uint32)
+        rays = np.zeros(n, np.uint32)
+        bands = np.zeros((pts.size, abi.LAYERS, 3, 8), np.float32) if point_bands else None
+        err = C.create_string_buffer(512)
+        rc = library().qm_evaluate_line_pairs(self.handle, square.handle, _ptr(pts), pts.size, _ptr(pcs), pcs.size,
+                                              _ptr(pp), _ptr(pl), n, _ptr(sums), _ptr(failed), _ptr(rays),
+                                              _ptr(bands), err, 512)
+        _check(rc, err)
+        return dict(sums=sums, failed=failed, rays=rays, point_bands=bands)
 
     def crossings(self, square: Square, segments, capacity: int = 48):
         seg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)

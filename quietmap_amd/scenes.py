@@ -448,3 +448,34 @@ def tile_mosaic(seed=4, tiles=(4, 4), receivers=256, spacing=10.0, source_median
     scene = abi.Scene(terrain, obstacles, weather)
     return TileMosaic(scene, np.concatenate(all_src), np.concatenate(pts), (receivers, receivers), (tx, ty),
                       np.array(first, np.uint32))
+
+
+def line_pieces(rng, count_lines, extent, origin, piece_m=250.0, height_m=0.05, level_mean=75.0, level_sd=6.0):
+    """Random polylines (2-5 vertices over the extent) cut into straight pieces of at most
+    `piece_m`, with an emission per metre of N(level_mean, level_sd) dB per band + A-weighting."""
+    out = []
+    for _ in range(count_lines):
+        k = int(rng.integers(2, 6))
+        verts = np.stack([origin[0] + rng.uniform(0, extent[0], k), origin[1] + rng.uniform(0, extent[1], k)], 1)
+        levels = rng.normal(level_mean, level_sd, 8) + A_WEIGHTING_DB
+        for a, b in zip(verts[:-1], verts[1:]):
+            length = float(np.hypot(*(b - a)))
+            parts = max(int(math.ceil(length / piece_m)), 1)
+            for j in range(parts):
+                p = np.zeros((), abi.LINE_PIECE_DTYPE)
+                p["ax_m"], p["ay_m"] = a + (b - a) * (j / parts)
+                p["bx_m"], p["by_m"] = a + (b - a) * ((j + 1) / parts)
+                p["height_m"] = height_m
+                p["energy_per_m"] = np.repeat((10.0 ** (levels / 10.0))[None, :], 3, axis=0)
+                out.append(p)
+    return np.array(out, abi.LINE_PIECE_DTYPE)
+
+
+def line_scene(seed=5, receivers=96, spacing=10.0, lines=30, buildings=300):
+    """The plain road scene's ground, buildings and weather with road polylines kept as line
+    pieces (<= 250 m) instead of points: the line-source rows' test scene."""
+    sc = road_scene(seed=seed, receivers=receivers, spacing=spacing, lines=1, per_line=1, buildings=buildings)
+    rng = np.random.default_rng(seed + 100)
+    extent = receivers * spacing
+    sc.pieces = line_pieces(rng, lines, (extent, extent), (0.0, 0.0))
+    return sc
