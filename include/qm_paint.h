@@ -94,12 +94,37 @@ int qm_loudness_device(qm_gpu* gpu, const float* d_levels, size_t n, size_t cell
  * flights' segments directly. Per receiver (`points_xy`, nx*ny x/y pairs, `receiver_altitude_m`
  * above the tracks' datum, every segment within `reach_m` horizontally): the SEL energy per period
  * summed over flights a day (`period_energy`, 3 doubles), the flights a day whose LAmax reaches
- * 65 and 70 dB, and `hist` (QM_HIST_BINS counts a receiver: flights per 0.5 dB of LAmax from 40 dB).
- * Host arrays. */
+ * 65 and 70 dB, `hist` (QM_HIST_BINS counts a receiver: flights per 0.5 dB of LAmax from 40 dB)
+ * and, unless null, `passes` (QM_PASS_FLOATS doubles a receiver): the pass classes of
+ * FlightPasses::add_pass (popup/src/aircraft.rs:86-95, the card's pass_class and flight_sums
+ * class sums, paint-gpu/kernels/aircraft.cuh:250-311), each heard segment a pass at its SEL energy
+ * over the period's seconds while its flights are on it (box_passing, boxes.rs:225-265, with the
+ * segment's 3-D length as the chord). Host arrays. */
 int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_classes, const qm_track* tracks,
                      size_t n_tracks, const float* vertices, size_t n_vertices, const double* points_xy, uint32_t nx,
                      uint32_t ny, double receiver_altitude_m, double reach_m, double* period_energy, uint32_t* n65,
-                     uint32_t* n70, uint32_t* hist, char* err, size_t errlen);
+                     uint32_t* n70, uint32_t* hist, double* passes, char* err, size_t errlen);
+
+/* Pixel loudness on the card (the reference's Gpu::pixel_loudness, paint-gpu/src/loudness.rs, over
+ * its kernels pixel_mixtures and pixel_ndens, paint-gpu/kernels/distribution.cuh and
+ * loudness.cuh): per packed pixel (paint/src/sone.rs pack: per layer its steady energy per
+ * period, its line count and each line's 43 floats; then per curve, the six layers' and the
+ * place's, and period 28 third-octave levels at 0 dB(A), the first NaN for none; pixel i from
+ * `offsets[i]` floats into `input`) the level distributions of every period, hour group and
+ * weather state at the map's 2 dB resolution, mixed per period (`mixtures`, optional:
+ * pixels x 3 periods x QM_LOUD_CURVES x QM_LOUD_BINS x 2 floats, a bin's share of the time and
+ * share x mean intensity), and Nden per curve (`ndens`, QM_LOUD_CURVES floats a pixel: each layer
+ * alone, then the place). `kurze` is the 41 x 100 quantile table (physics/src/percentile.rs
+ * TABLE); null takes qm_kurze_table's. Malformed counts fail the call. Host arrays. */
+#define QM_LOUD_BINS 90
+#define QM_LOUD_CURVES 7
+int qm_pixel_loudness(qm_gpu* gpu, const float* input, size_t input_floats, const uint64_t* offsets, size_t pixels,
+                      const float* kurze, float* mixtures, float* ndens, char* err, size_t errlen);
+
+/* The quantile table of Kurze's line statistics (physics/src/percentile_table.rs): the
+ * reference's seeded simulation (physics/src/percentile_tests.rs writes_the_table) restated, 41
+ * rows of lambda from 1e-4 at 8 a decade by 100 quantiles, FP32. */
+void qm_kurze_table(float* out);
 
 /* Gpu::evaluate_pairs for line candidates (paint-gpu/kernels/paint.cu:77-290 pair_rays, the
  * count / scan / list of rays, and ray_transfers; popup/src/evaluate.rs:171-267 source_rays): per

@@ -281,6 +281,22 @@ bool within_reach(const double s[2], const double e[2], double reach_m) {
     return px * px + py * py <= reach_m * reach_m;
 }
 
+// popup/src/aircraft.rs:47-95 FlightPasses::add_pass: the class of passes heard at `energy` (Leq,
+// linear) while `passing` of the period are in progress, 3 dB wide from -30 dB, 48 classes.
+int pass_class(double energy_leq, double passing) {
+    const double level_db = 10.0 * std::log10(energy_leq / passing);
+    const int c = (int)std::max((level_db - QM_PASS_LOWEST_DB) / QM_PASS_CLASS_DB, 0.0);
+    return std::min(c, QM_PASS_CLASSES - 1);
+}
+
+// popup/src/aircraft/boxes.rs:225-265 box_passing for one raw segment as the box: the period's
+// flights over its seconds, times the segment's 3-D length, times the seconds a metre at the track's
+// ground speed (the class's reference speed at or under 10 kt, corrections.rs:17).
+double seconds_per_metre(double speed_kt, double v_ref_kt) {
+    const double v = speed_kt > 10.0 ? speed_kt : v_ref_kt;
+    return 3600.0 / (v * 1852.0);
+}
+
 // The histogram bin of an event level, -1 below the floor.
 int hist_bin(double lmax_db) {
     if (!(lmax_db >= QM_HIST_FLOOR_DB)) return -1;
@@ -338,7 +354,8 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
                         const float* vertices, const double* points /* x, y pairs */, size_t n_points,
                         double receiver_altitude_m, double reach_m, double* energy_out /* 3 a point */,
                         uint32_t* n65, uint32_t* n70, uint32_t* hist /* QM_HIST_BINS a point */,
-                        uint64_t* segments_heard, int threads) {
+                        double* passes /* QM_PASS_FLOATS a point, or null */, uint64_t* segments_heard,
+                        int threads) {
     std::vector<Class> cls(n_classes);
     for (size_t k = 0; k < n_classes; ++k) {
         cls[k].installation = classes[k].installation;
@@ -360,6 +377,10 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
         uint64_t heard = 0;
         uint32_t* h = hist + i * QM_HIST_BINS;
         for (int b = 0; b < QM_HIST_BINS; ++b) h[b] = 0;
+        double* class_energy = passes ? passes + i * QM_PASS_FLOATS : nullptr;
+        double* class_passing = passes ? class_energy + 3 * QM_PASS_CLASSES : nullptr;
+        if (passes)
+            for (int k = 0; k < QM_PASS_FLOATS; ++k) class_energy[k] = 0.0;
         for (size_t t = 0; t < n_tracks; ++t) {
             const qm_track& tr = tracks[t];
             const Class& c = cls[tr.aircraft_class];
@@ -377,8 +398,26 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
                 any = true;
                 ++heard;
                 const Closest cp = closest_points(s, en);
-                track_energy += energy(segment_sel_db(em, cp));
+                const double sel_energy = energy(segment_sel_db(em, cp));
+                track_energy += sel_energy;
                 lmax = std::max(lmax, segment_lmax_db(em, cp));
+                if (passes) {
+                    // aircraft.cuh flight_sums: e = SEL energy / period seconds, passing share.
+                    const double d0 = (double)b[0] - (double)a[0], d1 = (double)b[1] - (double)a[1],
+                                 d2 = (double)b[2] - (double)a[2];
+                    const double length = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    const double spm = seconds_per_metre((double)tr.speed_kt, c.v_ref_kt);
+                    for (int p = 0; p < 3; ++p) {
+                        const double seconds = QM_PERIOD_HOURS[p] * 3600.0;
+                        const double e_p = (double)tr.flights[p] * sel_energy / seconds;
+                        const double passing = (double)tr.flights[p] / seconds * length * spm;
+                        if (e_p > 0.0 && passing > 0.0) {
+                            const int k = p * QM_PASS_CLASSES + pass_class(e_p, passing);
+                            class_energy[k] += e_p;
+                            class_passing[k] += passing;
+                        }
+                    }
+                }
             }
             if (!any) continue;
             for (int p = 0; p < 3; ++p) e[p] += (double)tr.flights[p] * track_energy;

@@ -72,6 +72,21 @@ def _declare(L):
     L.qmo_paint_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_size_t, C.c_double, D, U64, U32, C.c_int]
     L.qmo_zwicker.argtypes = [D, C.c_size_t, D]
+    L.qmo_kurze_table.argtypes = [F32]
+    L.qmo_relative_intensity.argtypes = [C.c_double, C.c_double]
+    L.qmo_relative_intensity.restype = C.c_double
+    L.qmo_node_probabilities.argtypes = [D, C.c_size_t]
+    L.qmo_node_probabilities.restype = C.c_size_t
+    L.qmo_line_values.argtypes = [C.c_int, C.c_double, D, C.c_size_t]
+    L.qmo_line_values.restype = C.c_size_t
+    L.qmo_runs.argtypes = [D, D, C.c_size_t, C.c_double, C.c_double, C.c_double, D]
+    L.qmo_runs.restype = C.c_int
+    L.qmo_many_lines_energy.argtypes = [C.c_double, C.c_int, C.c_double]
+    L.qmo_many_lines_energy.restype = C.c_double
+    L.qmo_line_exceeded_db.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.qmo_line_exceeded_db.restype = C.c_double
+    L.qmo_hour_groups.argtypes = [C.POINTER(C.c_int), D]
+    L.qmo_pixel_loudness.argtypes = [F32, U64, C.c_size_t, C.c_double, D, D, C.c_int]
     L.qmo_finite_fraction.argtypes = [C.c_double, C.c_double, C.c_double]
     L.qmo_finite_fraction.restype = C.c_double
     L.qmo_lateral_attenuation.argtypes = [C.c_double, C.c_double]
@@ -81,7 +96,7 @@ def _declare(L):
     L.qmo_closest_points.argtypes = [D, D, D]
     L.qmo_segment_levels.argtypes = [C.c_void_p, C.c_void_p, D, D, D, D]
     L.qmo_aircraft_points.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, D, C.c_size_t,
-                                      C.c_double, C.c_double, D, U32, U32, U32, U64, C.c_int]
+                                      C.c_double, C.c_double, D, U32, U32, U32, D, U64, C.c_int]
 
 
 def finite_fraction(along, length, scaled):
@@ -116,24 +131,29 @@ def segment_levels(classes, track, start, end):
     return sel[0], lmax[0]
 
 
-def aircraft_points(scene, points_xy, threads=None):
-    """Per receiver: SEL energy per period, N65, N70, LAmax histogram, segments heard."""
+def aircraft_points(scene, points_xy, threads=None, passes=False):
+    """Per receiver: SEL energy per period, N65, N70, LAmax histogram, segments heard, and with
+    `passes` the pass classes [n, 2, 3, 48] (energies, then passing shares)."""
     xy = np.ascontiguousarray(points_xy, np.float64).reshape(-1, 2)
     n = xy.shape[0]
     energy = np.zeros((n, 3))
     n65, n70 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
     hist = np.zeros((n, abi.HIST_BINS), np.uint32)
     heard = np.zeros(n, np.uint64)
+    pc = np.zeros((n, 2, 3, abi.PASS_CLASSES)) if passes else None
     classes = np.ascontiguousarray(scene.classes, abi.AIRCRAFT_CLASS_DTYPE)
     tracks = np.ascontiguousarray(scene.tracks, abi.TRACK_DTYPE)
     verts = np.ascontiguousarray(scene.vertices, np.float32)
     rc = lib().qmo_aircraft_points(classes.ctypes.data, classes.size, tracks.ctypes.data, tracks.size,
                                    verts.ctypes.data, _p(xy, D), n, scene.receiver_altitude_m, scene.reach_m,
-                                   _p(energy, D), _p(n65, U32), _p(n70, U32), _p(hist, U32), _p(heard, U64),
-                                   threads or os.cpu_count() or 1)
+                                   _p(energy, D), _p(n65, U32), _p(n70, U32), _p(hist, U32),
+                                   _p(pc, D) if passes else None, _p(heard, U64), threads or os.cpu_count() or 1)
     if rc != 0:
         raise ValueError("oracle: track out of its class's power rows")
-    return dict(period_energy=energy, n65=n65, n70=n70, hist=hist, segments=heard)
+    out = dict(period_energy=energy, n65=n65, n70=n70, hist=hist, segments=heard)
+    if passes:
+        out["passes"] = pc
+    return out
 
 
 def zwicker_sone(levels):
@@ -362,3 +382,72 @@ def paint_points(scene: abi.Scene, points, sources, cutoff_m, threads=None):
                            points.ctypes.data, n, sources.ctypes.data, sources.size, cutoff_m, _p(energy, D),
                            _p(paths, U64), _p(failures, U32), threads)
     return dict(period_energy=energy, paths=paths, failures=failures)
+
+
+# ---------------------------------------------------------------------------------------------
+# Level distributions and pixel loudness (distribution.cpp).
+
+def kurze_table():
+    """The seeded quantile table, (41, 100) float32."""
+    t = np.zeros(41 * 100, np.float32)
+    lib().qmo_kurze_table(_p(t, F32))
+    return t.reshape(41, 100)
+
+
+def relative_intensity(lam, p):
+    return lib().qmo_relative_intensity(float(lam), float(p))
+
+
+def node_probabilities():
+    out = np.zeros((200, 2))
+    n = lib().qmo_node_probabilities(_p(out, D), 200)
+    return out[:n]
+
+
+def line_values(kind, x):
+    """kind "passes" (mean in progress), "kurze" (lambda) or "event" (duty): (value, share) rows."""
+    out = np.zeros((512, 2))
+    n = lib().qmo_line_values({"passes": 0, "kurze": 1, "event": 2}[kind], float(x), _p(out, D), 512)
+    assert n <= 512
+    return out[:n]
+
+
+def runs(values, shares, scale, intensity, bin_db):
+    """Distribution::runs: per bin (shares, weighted), the last row silence; and whether a bin was
+    visited twice."""
+    v = np.ascontiguousarray(values, np.float64)
+    s = np.ascontiguousarray(shares, np.float64)
+    bins = int(round(180.0 / bin_db))
+    out = np.zeros((bins + 1, 2))
+    twice = lib().qmo_runs(_p(v, D), _p(s, D), v.size, float(scale), float(intensity), float(bin_db), _p(out, D))
+    return out, bool(twice)
+
+
+def many_lines_energy(lam, count, bin_db):
+    return lib().qmo_many_lines_energy(float(lam), int(count), float(bin_db))
+
+
+def line_exceeded_db(lam, exceeded, bin_db):
+    return lib().qmo_line_exceeded_db(float(lam), float(exceeded), float(bin_db))
+
+
+def hour_groups():
+    """The map's hour groups: slot counts (3, 4) and factors (3 profiles, 3 periods, 4 groups)."""
+    slots = np.zeros(12, np.int32)
+    factors = np.zeros(36)
+    lib().qmo_hour_groups(slots.ctypes.data_as(C.POINTER(C.c_int)), _p(factors, D))
+    return slots.reshape(3, 4), factors.reshape(3, 3, 4)
+
+
+def pixel_loudness(packed, offsets, bin_db=2.0, mixtures=False, threads=None):
+    """Per pixel Nden per curve (7: six layers, then the place) and optionally the mixtures
+    (pixels, 3 periods, 7 curves, bins, 2)."""
+    packed = np.ascontiguousarray(packed, np.float32)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    n = offsets.size
+    bins = int(round(180.0 / bin_db))
+    nd = np.zeros((n, 7))
+    mix = np.zeros((n, 3, 7, bins, 2)) if mixtures else None
+    lib().qmo_pixel_loudness(_p(packed, F32), _p(offsets, U64), n, float(bin_db), _p(mix, D) if mixtures else None,
+                             _p(nd, D), threads or os.cpu_count() or 1)
+    return (nd, mix) if mixtures else nd

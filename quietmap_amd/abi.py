@@ -207,6 +207,10 @@ INSTALL_WING, INSTALL_FUSELAGE, INSTALL_PROPELLER = 0, 1, 2
 HIST_BINS = 160
 HIST_FLOOR_DB = 40.0
 HIST_STEP_DB = 0.5
+PASS_CLASSES = 48
+PASS_CLASS_DB = 3.0
+PASS_LOWEST_DB = -30.0
+PASS_FLOATS = 2 * 3 * PASS_CLASSES
 AIRCRAFT_CLASS_DTYPE = np.dtype(
     [
         ("sel_db", "<f8", (2, POWER_ROWS, NPD_DISTANCES)),

@@ -1,6 +1,6 @@
 // Doc 29 aircraft noise at a receiver grid from raw flight tracks: per receiver the SEL energy of
-// every flight per period (Lden), the flights a day whose LAmax reaches 65 and 70 dB, and an integer
-// histogram of flights per 0.5 dB of LAmax.
+// every flight per period (Lden), the flights a day whose LAmax reaches 65 and 70 dB, an integer
+// histogram of flights per 0.5 dB of LAmax and, optionally, the 48 pass classes per period.
 //
 // Reference behaviour restated: physics/src/doc29/segment.rs:200-381 (closest points, the
 // segment's SEL and LAmax), npd.rs:121-330 (NPD positions, power rows, tails),
@@ -44,6 +44,8 @@ struct DevRow {
     uint32_t pad;
     double lamax_d[ND];
     double tail_alpha_d;
+    double sel_d[ND], sml_d[ND];
+    double d0_d;
 };
 
 // A track with everything that is constant along it.
@@ -56,6 +58,8 @@ struct DevTrack {
     float sel_last, sel_alpha;  // the SEL tail past 25,000 ft (segment.rs:177-190)
     float lamax_last, lamax_alpha;
     double weight_d, lamax_last_d, lamax_alpha_d;
+    double sel_last_d, sel_alpha_d, dv_d;
+    double seconds_per_metre;  // of a pass (boxes.rs:245-257)
     uint32_t flights[3];
     uint32_t total_flights;
 };
@@ -77,6 +81,8 @@ struct AircraftArgs {
     uint32_t* n65;
     uint32_t* n70;
     uint32_t* hist;
+    double* passes;  // QM_PASS_FLOATS a receiver, or null
+    double seconds[3];
     uint32_t* counters;
     Tables tables;
 };
@@ -97,9 +103,32 @@ __device__ __forceinline__ void position_f(const Tables& T, float slant, int& in
 
 __device__ __forceinline__ float linear_f(const float* c, int i, float f) { return c[i] + f * (c[i + 1] - c[i]); }
 
+// corrections.rs:36-45: (e(a1) - e(a0)) / pi with e(a) = a / (1 + a^2) + atan(a), a1 - a0 =
+// length / scaled. Where both ends lie two or more scaled distances to one side, e is within 0.03
+// of pi / 2 at both and the FP32 difference loses its digits (a segment far behind the receiver
+// read the 1e-15 floor for a fraction of 1e-8, which moved its pass to the lowest class); there
+// the integral of e' = 2 / (1 + a^2)^2 is taken in u = 1 / |a|, 2 u^2 / (1 + u^2)^2, whose
+// antiderivative sum_k (-1)^(k+1) 2k / (2k+1) u^(2k+1) gives the difference term by term as
+// (u0 - u1) sum_k c_k P_2k, P_n = (u0^(n+1) - u1^(n+1)) / (u0 - u1), u <= 1/2.
 __device__ __forceinline__ float finite_fraction_f(float along, float length, float scaled) {
     if (length < 1.0f || scaled < 1.0f) return 1.0f;
     const float a0 = -along / scaled, a1 = -(along - length) / scaled;
+    if (a0 * a1 > 0.0f && fminf(fabsf(a0), fabsf(a1)) >= 2.0f) {
+        const float u0 = 1.0f / fminf(fabsf(a0), fabsf(a1)), u1 = 1.0f / fmaxf(fabsf(a0), fabsf(a1));
+        const float du = (length / scaled) / (a0 * a1);
+        float p = 1.0f, power = 1.0f, sum = 0.0f, sign = 1.0f;
+#pragma unroll
+        for (int n = 1; n <= 28; ++n) {
+            power *= u0;
+            p = u1 * p + power;
+            if ((n & 1) == 0) {
+                const float k = (float)(n / 2);
+                sum += sign * (2.0f * k / (2.0f * k + 1.0f)) * p;
+                sign = -sign;
+            }
+        }
+        return fmaxf(du * sum * (float)(1.0 / PI_D), 1e-15f);
+    }
     const float e0 = a0 / (1.0f + a0 * a0) + atanf(a0);
     const float e1 = a1 / (1.0f + a1 * a1) + atanf(a1);
     return fmaxf((e1 - e0) * (float)(1.0 / PI_D), 1e-15f);
@@ -210,6 +239,106 @@ __device__ double segment_lmax_d(const AircraftArgs& g, const DevTrack& tr, cons
     const double lat = (along >= 0.0 && along <= 1.0) ? lateral_d(height, lateral)
                                                       : lateral_d(on_seg[2], hypot(on_seg[0], on_seg[1]));
     return lamax + 10.0 * log10(installation_d(tr.installation, height, line_slant)) - lat;
+}
+
+// 10^(level / 10) as the oracle takes it (bands.rs:36-38).
+__device__ double energy_d(double level_db) {
+#pragma clang fp contract(off)
+    return exp(level_db * (2.302585092994045684017991454684364208 / 10.0));
+}
+
+__device__ double finite_fraction_d(double along, double length, double scaled) {
+#pragma clang fp contract(off)
+    if (length < 1.0 || scaled < 1.0) return 1.0;
+    const double a0 = -along / scaled, a1 = -(along - length) / scaled;
+    const double e0 = a0 / (1.0 + a0 * a0) + atan(a0);
+    const double e1 = a1 / (1.0 + a1 * a1) + atan(a1);
+    return fmax((e1 - e0) * (1.0 / PI_D), 1e-15);
+}
+
+// One power row's SEL and scaled distance at an NPD position (npd.rs:290-330 PowerRow::read).
+__device__ void row_read_d(const DevRow& r, int interval, double fraction, double slant, double& sel,
+                           double& scaled) {
+#pragma clang fp contract(off)
+    if (r.dipole) {
+        scaled = slant;
+    } else {
+        const double lower = r.sml_d[interval];
+        scaled = r.d0_d * energy_d(lower + fraction * (r.sml_d[interval + 1] - lower));
+    }
+    if (slant < NPD_LAST_DISTANCE_M) {
+        const double lower = r.sel_d[interval];
+        sel = lower + fraction * (r.sel_d[interval + 1] - lower);
+    } else {
+        sel = r.sel_d[ND - 1] - 20.0 * log10(slant / NPD_LAST_DISTANCE_M) - r.tail_alpha_d * (slant - NPD_LAST_DISTANCE_M);
+    }
+}
+
+// The SEL (dB) of segment (a, b) in FP64, in the oracle's operation order (segment.rs:318-350 over
+// the closest points, SegmentEmission::read_npd and corrections.rs), unscreened.
+__device__ __noinline__ double segment_sel_d(const AircraftArgs& g, const DevTrack& tr, const DevRow* rows, const float* a,
+                                const float* b, double rx, double ry) {
+#pragma clang fp contract(off)
+    const double s[3] = {(double)a[0] - rx, (double)a[1] - ry, (double)a[2] - g.receiver_altitude_m};
+    const double e[3] = {(double)b[0] - rx, (double)b[1] - ry, (double)b[2] - g.receiver_altitude_m};
+    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    const double hs = d[0] * d[0] + d[1] * d[1];
+    const double ls = hs + d[2] * d[2];
+    double along, on_line[3], lateral, height, lambda_height;
+    if (hs <= 1e-6) {
+        along = 0.0;
+        for (int k = 0; k < 3; ++k) on_line[k] = s[k];
+        lateral = hypot(s[0], s[1]);
+        height = s[2];
+        lambda_height = s[2];
+    } else {
+        along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / ls;
+        const double cl = fmin(fmax(along, 0.0), 1.0);
+        double on_seg2 = 0.0;
+        for (int k = 0; k < 3; ++k) on_line[k] = s[k] + along * d[k];
+        on_seg2 = s[2] + cl * d[2];
+        lateral = fabs(s[0] * d[1] - s[1] * d[0]) / sqrt(hs);
+        const double ss = on_line[0] * on_line[0] + on_line[1] * on_line[1] + on_line[2] * on_line[2];
+        height = copysign(sqrt(fmax(ss - lateral * lateral, 0.0)), on_line[2]);
+        const double cos_climb = sqrt(hs / ls);
+        lambda_height = (along >= 0.0 && along <= 1.0) ? height : on_seg2 / cos_climb;
+    }
+    const double length = fmax(sqrt(ls), 1.0);
+    const double slant_in = hypot(hypot(on_line[0], on_line[1]), on_line[2]);
+    // SegmentEmission::read_npd (segment.rs:126-146): the position, the power-interpolated table.
+    const double slant = fmax(slant_in, NPD_NEAREST_SLANT_M);
+    const double log_d = log10(slant / METRES_PER_FOOT);
+    int interval = 0;
+    for (int k = ND - 2; k >= 1; --k)
+        if (log_d > g.tables.log_npd_d[k]) {
+            interval = k;
+            break;
+        }
+    const double fraction =
+        (log_d - g.tables.log_npd_d[interval]) / (g.tables.log_npd_d[interval + 1] - g.tables.log_npd_d[interval]);
+    double sel, scaled;
+    row_read_d(rows[tr.row_index], interval, fraction, slant, sel, scaled);
+    if (tr.weight_d != 0.0) {
+        double sel_hi, scaled_hi;
+        row_read_d(rows[tr.row_index + 1], interval, fraction, slant, sel_hi, scaled_hi);
+        sel = sel + tr.weight_d * (sel_hi - sel);
+        scaled = scaled + tr.weight_d * (scaled_hi - scaled);
+    }
+    if (!(slant < NPD_LAST_DISTANCE_M))
+        sel = tr.sel_last_d - 20.0 * log10(slant / NPD_LAST_DISTANCE_M) - tr.sel_alpha_d * (slant - NPD_LAST_DISTANCE_M);
+    const double finite = 10.0 * log10(finite_fraction_d(along * length, length, scaled));
+    const double lat = lateral_d(lambda_height, lateral);
+    const double installation = 10.0 * log10(installation_d(tr.installation, height, slant_in));
+    const double free_sel = sel + tr.dv_d + installation - lat + finite;
+    return free_sel - fmax(0.0 - lat, 0.0);
+}
+
+// FlightPasses::add_pass's class in FP64 (popup/src/aircraft.rs:86-95).
+__device__ int pass_class_d(double energy, double passing) {
+#pragma clang fp contract(off)
+    const double level_db = 10.0 * log10(energy / passing);
+    const int c = (int)fmax((level_db - QM_PASS_LOWEST_DB) / QM_PASS_CLASS_DB, 0.0);
+    return min(c, QM_PASS_CLASSES - 1);
 }
 
 // The reach rule in FP64 (the oracle's within_reach).
@@ -331,6 +460,9 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
 
 }  // namespace
 
+// PASSES: the pass classes are summed (g.passes is not null); the kernel without them keeps the
+// registers of the energies, counts and histogram alone.
+template <bool PASSES>
 __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
     __shared__ uint32_t s_tile;
     const int lane = threadIdx.x;
@@ -352,6 +484,19 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
         uint32_t* hist = g.hist + index * QM_HIST_BINS;
         if (valid)
             for (int k = 0; k < QM_HIST_BINS; ++k) hist[k] = 0u;
+        // Pass classes: a run of one class per period sums in registers and is added to the
+        // receiver's row when the class changes.
+        double* pass_row = (PASSES && valid) ? g.passes + index * QM_PASS_FLOATS : nullptr;
+        if (pass_row)
+            for (int k = 0; k < QM_PASS_FLOATS; ++k) pass_row[k] = 0.0;
+        int run_class[3] = {-1, -1, -1};
+        double run_energy[3] = {0.0, 0.0, 0.0}, run_passing[3] = {0.0, 0.0, 0.0};
+        auto flush = [&](int p) {
+            if (run_class[p] < 0) return;
+            pass_row[p * QM_PASS_CLASSES + run_class[p]] += run_energy[p];
+            pass_row[(3 + p) * QM_PASS_CLASSES + run_class[p]] += run_passing[p];
+            run_energy[p] = run_passing[p] = 0.0;
+        };
         // The tile's box (uniform).
         const uint32_t ix0 = tx * 16, iy0 = ty * 16;
         const uint32_t ix1 = min(ix0 + 15, g.nx - 1), iy1 = min(iy0 + 15, g.ny - 1);
@@ -376,6 +521,36 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
                     if (sf.lmax == -INFINITY) continue;
                     any = true;
                     e_track += sf.sel_energy;
+                    if (PASSES && pass_row) {
+                        // The pass's class (aircraft.cuh flight_sums, pass_class): the FP32 level
+                        // decides it unless within EPS_DB of a class edge, where the FP64 SEL does.
+                        const float* pa = v + 3 * k;
+                        const double d0 = (double)pa[3] - (double)pa[0], d1 = (double)pa[4] - (double)pa[1],
+                                     d2 = (double)pa[5] - (double)pa[2];
+                        const double length = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                        const float level = 10.0f * log10f(sf.sel_energy / (float)(length * tr.seconds_per_metre));
+                        const float position = (level - (float)QM_PASS_LOWEST_DB) / (float)QM_PASS_CLASS_DB;
+                        const int fast_class = min((int)fmaxf(position, 0.0f), QM_PASS_CLASSES - 1);
+                        const float edge = rintf(position);
+                        const bool near_edge =
+                            edge >= 1.0f && edge <= (float)(QM_PASS_CLASSES - 1) &&
+                            fabsf(level - ((float)QM_PASS_LOWEST_DB + (float)QM_PASS_CLASS_DB * edge)) < EPS_DB;
+                        const double exact = near_edge ? energy_d(segment_sel_d(g, tr, g.rows, pa, pa + 3, rx, ry)) : 0.0;
+                        for (int p = 0; p < 3; ++p) {
+                            const double seconds = g.seconds[p];
+                            const double e_p = (double)tr.flights[p] * (double)sf.sel_energy / seconds;
+                            const double passing = (double)tr.flights[p] / seconds * length * tr.seconds_per_metre;
+                            if (!(e_p > 0.0 && passing > 0.0)) continue;
+                            const int c = near_edge ? pass_class_d((double)tr.flights[p] * exact / seconds, passing)
+                                                    : fast_class;
+                            if (c != run_class[p]) {
+                                flush(p);
+                                run_class[p] = c;
+                            }
+                            run_energy[p] += e_p;
+                            run_passing[p] += passing;
+                        }
+                    }
                     if (sf.lmax > m1) {
                         m3 = m2, i3 = i2, m2 = m1, i2 = i1, m1 = sf.lmax, i1 = k;
                     } else if (sf.lmax > m2) {
@@ -408,6 +583,8 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
                 hist[bin] += tr.total_flights;
             }
         }
+        if (PASSES && pass_row)
+            for (int p = 0; p < 3; ++p) flush(p);
         if (valid) {
             for (int p = 0; p < 3; ++p) g.energy[index * 3 + p] = energy[p];
             g.n65[index] = n65;
@@ -494,8 +671,8 @@ double spherical(double s) { return 20.0 * std::log10(s / NPD_LAST_DISTANCE_M); 
 extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_classes, const qm_track* tracks,
                                 size_t n_tracks, const float* vertices, size_t n_vertices, const double* points_xy,
                                 uint32_t nx, uint32_t ny, double receiver_altitude_m, double reach_m,
-                                double* period_energy, uint32_t* n65, uint32_t* n70, uint32_t* hist, char* err,
-                                size_t errlen) {
+                                double* period_energy, uint32_t* n65, uint32_t* n70, uint32_t* hist,
+                                double* passes, char* err, size_t errlen) {
     if (!gpu || (n_tracks && (!classes || !tracks || !vertices)) || !points_xy || !period_energy || !n65 || !n70 || !hist)
         return afail(err, errlen, "null argument");
     const size_t n = (size_t)nx * ny;
@@ -521,7 +698,10 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
                     d.lamax[k] = (float)h.lamax[k];
                     d.sml[k] = (float)h.sml[k];
                     d.lamax_d[k] = h.lamax[k];
+                    d.sel_d[k] = h.sel[k];
+                    d.sml_d[k] = h.sml[k];
                 }
+                d.d0_d = h.d0;
                 d.tail_alpha = (float)h.tail_alpha;
                 d.tail_alpha_d = h.tail_alpha;
                 d.d0 = (float)h.d0;
@@ -561,7 +741,9 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
         d.installation = c.installation;
         d.weight = tr.power_weight;
         d.weight_d = (double)tr.power_weight;
-        d.dv = (float)(tr.speed_kt > 10.0f ? 10.0 * std::log10(c.v_ref_kt / (double)tr.speed_kt) : 0.0);
+        d.dv_d = (double)tr.speed_kt > 10.0 ? 10.0 * std::log10(c.v_ref_kt / (double)tr.speed_kt) : 0.0;
+        d.dv = (float)d.dv_d;
+        d.seconds_per_metre = 3600.0 / (((double)tr.speed_kt > 10.0 ? (double)tr.speed_kt : c.v_ref_kt) * 1852.0);
         // table(25,000 ft) and table(16 km) of SEL and LAmax, power-interpolated (segment.rs:126-162).
         const HostRow& lo = host_rows[d.row_index];
         auto at = [&](const double* lo_curve, const double* hi_curve, double lo_alpha, double hi_alpha, double s) {
@@ -582,6 +764,8 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
         d.lamax_alpha = (float)lam_alpha;
         d.lamax_last_d = lam_last;
         d.lamax_alpha_d = lam_alpha;
+        d.sel_last_d = sel_last;
+        d.sel_alpha_d = sel_alpha;
         for (int p = 0; p < 3; ++p) d.flights[p] = tr.flights[p];
         d.total_flights = tr.flights[0] + tr.flights[1] + tr.flights[2];
         dev_tracks[t] = d;
@@ -596,9 +780,9 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
         g.tables.log_npd[k] = (float)g.tables.log_npd_d[k];
     }
     void *d_rows = nullptr, *d_tracks = nullptr, *d_vertices = nullptr, *d_points = nullptr, *d_energy = nullptr,
-         *d_n65 = nullptr, *d_n70 = nullptr, *d_hist = nullptr, *d_counters = nullptr;
+         *d_n65 = nullptr, *d_n70 = nullptr, *d_hist = nullptr, *d_counters = nullptr, *d_passes = nullptr;
     auto cleanup = [&]() {
-        for (void* p : {d_rows, d_tracks, d_vertices, d_points, d_energy, d_n65, d_n70, d_hist, d_counters})
+        for (void* p : {d_rows, d_tracks, d_vertices, d_points, d_energy, d_n65, d_n70, d_hist, d_counters, d_passes})
             if (p) (void)hipFree(p);
     };
     hipError_t e = hipSuccess;
@@ -614,6 +798,7 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
     alloc(&d_n70, sizeof(uint32_t) * n);
     alloc(&d_hist, sizeof(uint32_t) * QM_HIST_BINS * n);
     alloc(&d_counters, sizeof(uint32_t) * qm::XCDS);
+    if (passes) alloc(&d_passes, sizeof(double) * QM_PASS_FLOATS * n);
     if (e == hipSuccess && !rows.empty()) e = hipMemcpy(d_rows, rows.data(), sizeof(DevRow) * rows.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && n_tracks) e = hipMemcpy(d_tracks, dev_tracks.data(), sizeof(DevTrack) * n_tracks, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_vertices) e = hipMemcpy(d_vertices, vertices, sizeof(float) * 3 * n_vertices, hipMemcpyHostToDevice);
@@ -637,19 +822,28 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
     g.n65 = (uint32_t*)d_n65;
     g.n70 = (uint32_t*)d_n70;
     g.hist = (uint32_t*)d_hist;
+    g.passes = (double*)d_passes;
+    for (int p = 0; p < 3; ++p) g.seconds[p] = QM_PERIOD_HOURS[p] * 3600.0;
     g.counters = (uint32_t*)d_counters;
     hipLaunchKernelGGL(qm::zero_u32_kernel, dim3(1), dim3(64), 0, 0, g.counters, (uint32_t)qm::XCDS);
     int cus = 0, per_cu = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(aircraft_kernel), 256, 0);
+    const void* kernel = passes ? reinterpret_cast<const void*>(aircraft_kernel<true>)
+                                : reinterpret_cast<const void*>(aircraft_kernel<false>);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
     unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
     groups = std::min<unsigned>(groups, g.n_tiles);
-    hipLaunchKernelGGL(aircraft_kernel, dim3(groups), dim3(256), 0, 0, g);
+    if (passes)
+        hipLaunchKernelGGL(aircraft_kernel<true>, dim3(groups), dim3(256), 0, 0, g);
+    else
+        hipLaunchKernelGGL(aircraft_kernel<false>, dim3(groups), dim3(256), 0, 0, g);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(period_energy, d_energy, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(n65, d_n65, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(n70, d_n70, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(hist, d_hist, sizeof(uint32_t) * QM_HIST_BINS * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && passes)
+        e = hipMemcpy(passes, d_passes, sizeof(double) * QM_PASS_FLOATS * n, hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return afail(err, errlen, "aircraft_kernel: %s", hipGetErrorString(e));
     return 0;

@@ -24,6 +24,18 @@
 #define QM_HIST_FLOOR_DB 40.0
 #define QM_HIST_STEP_DB 0.5
 
+/* Pass classes (popup/src/aircraft.rs:47-53): per period, 48 classes of 3 dB from -30 dB of the
+ * level heard while a flight is on a segment, each the classes' summed energy (Leq, linear) and
+ * their summed share of the period in progress: QM_PASS_FLOATS doubles a receiver, the energies
+ * [period][class] then the passing shares [period][class]. */
+#define QM_PASS_CLASSES 48
+#define QM_PASS_CLASS_DB 3.0
+#define QM_PASS_LOWEST_DB (-30.0)
+#define QM_PASS_FLOATS (2 * 3 * QM_PASS_CLASSES)
+#ifdef __cplusplus
+static const double QM_PERIOD_HOURS[3] = {12.0, 4.0, 8.0};
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
 // blocks of 32x32 MFMA accumulators): a lane then holds one row and runs of four consecutive
 // features, which the epilogue converts in pairs (v_cvt_pk_bf16_f32) and stores 8 bytes at a time.
 // Weights are read from L2 a chunk of four K steps ahead; GELU is z * sigmoid(2u), the tanh
-// form's identity, with the hardware exp and reciprocal.
+// form's identity, with the hardware exp2 and reciprocal.
 #ifndef QM_INFER_ROWS
 #define QM_INFER_ROWS 128
 #endif
@@ -193,9 +193,14 @@ constexpr int RB = INFER_ROWS / 32;
 
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
+// exp(-2u) = 2^(z (k + k A z^2)) with k = -2 sqrt(2/pi) log2(e): one multiply, one FMA and one
+// multiply before the hardware exp2.
+constexpr float GELU_K = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float GELU_KA = GELU_K * 0.044715f;
+
 __device__ __forceinline__ float gelu_infer(float z) {
-    const float u = GELU_C * (z + GELU_A * z * z * z);
-    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(z * z, GELU_KA, GELU_K));
+    return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 // One hidden layer's products for a wave: features f0 .. f0+63 (two 32-feature blocks) over all
@@ -203,7 +208,8 @@ __device__ __forceinline__ float gelu_infer(float z) {
 // chunk's products run (the scheduling barriers keep the loads a whole chunk ahead).
 template <int K>
 __device__ __forceinline__ void hidden_products(const uint16_t (*act)[INFER_PITCH], const uint16_t* W0,
-                                                const uint16_t* W1, int lr, int lk, f32x16_t (&acc)[2][RB]) {
+                                                const uint16_t* W1, const float* bias, int lr, int lk,
+                                                f32x16_t (&acc)[2][RB]) {
     constexpr int STEPS = K / 16;
     constexpr int CHUNK = STEPS < 4 ? STEPS : 4;
     constexpr int CHUNKS = STEPS / CHUNK;
@@ -214,6 +220,19 @@ __device__ __forceinline__ void hidden_products(const uint16_t (*act)[INFER_PITC
         w[0][i][0] = *reinterpret_cast<const bf16x8_t*>(W0 + 16 * i);
         w[0][i][1] = *reinterpret_cast<const bf16x8_t*>(W1 + 16 * i);
     }
+    // The bias enters as the first step's accumulator input: element r of block fb is feature
+    // 32 fb + 8 (r / 4) + r % 4 of the lane's run (bias already offset to the run's start).
+    f32x16_t b[2];
+#pragma unroll
+    for (int fb = 0; fb < 2; ++fb)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float4 v = *reinterpret_cast<const float4*>(bias + 32 * fb + 8 * g);
+            b[fb][4 * g + 0] = v.x;
+            b[fb][4 * g + 1] = v.y;
+            b[fb][4 * g + 2] = v.z;
+            b[fb][4 * g + 3] = v.w;
+        }
 #pragma unroll
     for (int c = 0; c < CHUNKS; ++c) {
         if (c + 1 < CHUNKS) {
@@ -230,8 +249,8 @@ __device__ __forceinline__ void hidden_products(const uint16_t (*act)[INFER_PITC
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
                 const bf16x8_t x = *reinterpret_cast<const bf16x8_t*>(&act[rb * 32 + lr][16 * st + lk]);
-                acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][0], x, acc[0][rb], 0, 0, 0);
-                acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][1], x, acc[1][rb], 0, 0, 0);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][0], x, st ? acc[0][rb] : b[0], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[c & 1][i][1], x, st ? acc[1][rb] : b[1], 0, 0, 0);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -260,25 +279,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_INFER_WA
         const int f0 = wave * 64;
         const uint16_t* W0 = wb + LAYOUT.w[l] + (size_t)(f0 + lr) * fan_in + lk;
         const uint16_t* W1 = W0 + (size_t)32 * fan_in;
-        f32x16_t acc[2][RB] = {};
-        if (l == 0) hidden_products<K0>(act, W0, W1, lr, lk, acc);
-        else hidden_products<QM_MLP_HIDDEN>(act, W0, W1, lr, lk, acc);
+        f32x16_t acc[2][RB];
+        const float* bias = w + LAYOUT.b[l] + f0 + lq;
+        if (l == 0) hidden_products<K0>(act, W0, W1, bias, lr, lk, acc);
+        else hidden_products<QM_MLP_HIDDEN>(act, W0, W1, bias, lr, lk, acc);
         __syncthreads();  // every wave has read this layer's inputs
-        const float* bias = w + LAYOUT.b[l];
 #pragma unroll
         for (int fb = 0; fb < 2; ++fb)
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                // Features f0 + 32 fb + 8 g + lq .. +3 of row (rb, lr).
+                // Features f0 + 32 fb + 8 g + lq .. +3 of row (rb, lr), bias included.
                 const int feature = f0 + fb * 32 + 8 * g + lq;
-                const float4 bc = *reinterpret_cast<const float4*>(bias + feature);
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) {
                     bf16x4_t v;
-                    v.x = (__bf16)gelu_infer(acc[fb][rb][4 * g + 0] + bc.x);
-                    v.y = (__bf16)gelu_infer(acc[fb][rb][4 * g + 1] + bc.y);
-                    v.z = (__bf16)gelu_infer(acc[fb][rb][4 * g + 2] + bc.z);
-                    v.w = (__bf16)gelu_infer(acc[fb][rb][4 * g + 3] + bc.w);
+                    v.x = (__bf16)gelu_infer(acc[fb][rb][4 * g + 0]);
+                    v.y = (__bf16)gelu_infer(acc[fb][rb][4 * g + 1]);
+                    v.z = (__bf16)gelu_infer(acc[fb][rb][4 * g + 2]);
+                    v.w = (__bf16)gelu_infer(acc[fb][rb][4 * g + 3]);
                     *reinterpret_cast<bf16x4_t*>(&act[rb * 32 + lr][feature]) = v;
                 }
             }

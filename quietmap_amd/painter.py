@@ -37,6 +37,8 @@ EXPORTS = (
     "qm_aircraft_grid",
     "qm_crossings",
     "qm_evaluate_line_pairs",
+    "qm_pixel_loudness",
+    "qm_kurze_table",
     "qm_mlp_new",
     "qm_mlp_params",
     "qm_mlp_free",
@@ -84,7 +86,11 @@ def library():
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
         lib.qm_evaluate_line_pairs.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, sz, vp, vp, vp, vp, cp, sz]
-        lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, cp, sz]
+        lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, vp, cp,
+                                         sz]
+        lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
+        lib.qm_kurze_table.argtypes = [vp]
+        lib.qm_kurze_table.restype = None
         lib.qm_mlp_new.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
         lib.qm_mlp_params.argtypes = [vp, vp, cp, sz]
         lib.qm_mlp_free.argtypes = [vp]
@@ -230,9 +236,10 @@ class Gpu:
                                           stream or None, err, 512)
         _check(rc, err)
 
-    def aircraft_grid(self, scene):
+    def aircraft_grid(self, scene, passes=False):
         """Doc 29 aircraft outputs of an AircraftScene's receiver grid: SEL energy per period,
-        N65, N70 and the LAmax histogram per receiver."""
+        N65, N70 and the LAmax histogram per receiver, and with `passes` the pass classes
+        [n, 2, 3, 48] (energies, then passing shares, per period and class)."""
         nx, ny = scene.grid_shape
         xy = np.ascontiguousarray(scene.points_xy, np.float64).reshape(-1, 2)
         assert xy.shape[0] == nx * ny
@@ -243,12 +250,34 @@ class Gpu:
         energy = np.zeros((n, 3))
         n65, n70 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
         hist = np.zeros((n, abi.HIST_BINS), np.uint32)
+        pc = np.zeros((n, 2, 3, abi.PASS_CLASSES)) if passes else None
         err = C.create_string_buffer(512)
         rc = library().qm_aircraft_grid(self.handle, _ptr(classes), classes.size, _ptr(tracks), tracks.size,
                                         _ptr(verts), verts.shape[0], _ptr(xy), nx, ny, scene.receiver_altitude_m,
-                                        scene.reach_m, _ptr(energy), _ptr(n65), _ptr(n70), _ptr(hist), err, 512)
+                                        scene.reach_m, _ptr(energy), _ptr(n65), _ptr(n70), _ptr(hist),
+                                        _ptr(pc) if passes else None, err, 512)
         _check(rc, err)
-        return dict(period_energy=energy, n65=n65, n70=n70, hist=hist)
+        out = dict(period_energy=energy, n65=n65, n70=n70, hist=hist)
+        if passes:
+            out["passes"] = pc
+        return out
+
+    def pixel_loudness(self, packed, offsets, kurze=None, mixtures=False):
+        """Nden per curve (pixels, 7: each layer alone, then the place) of packed pixels
+        (quietmap_amd.pixels), and with `mixtures` their per-period mixtures
+        (pixels, 3, 7, 90, 2)."""
+        packed = np.ascontiguousarray(packed, np.float32)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        n = offsets.size
+        nd = np.zeros((n, 7), np.float32)
+        mix = np.zeros((n, 3, 7, 90, 2), np.float32) if mixtures else None
+        table = None if kurze is None else np.ascontiguousarray(kurze, np.float32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_pixel_loudness(self.handle, _ptr(packed), packed.size, _ptr(offsets), n,
+                                         _ptr(table) if table is not None else None,
+                                         _ptr(mix) if mixtures else None, _ptr(nd), err, 512)
+        _check(rc, err)
+        return (nd, mix) if mixtures else nd
 
     def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False):
         """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands."""
@@ -293,3 +322,10 @@ def lden_db(period_energy):
     total = (e * hours * penalty).sum(axis=-1) / 24.0
     with np.errstate(divide="ignore"):
         return 10.0 * np.log10(total)
+
+
+def kurze_table():
+    """The library's quantile table of Kurze's line statistics, (41, 100) float32 (host code)."""
+    t = np.zeros(41 * 100, np.float32)
+    library().qm_kurze_table(_ptr(t))
+    return t.reshape(41, 100)

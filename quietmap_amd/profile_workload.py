@@ -6,7 +6,9 @@
 in one paint_kernel launch (the defaults are the benchmark's step), `surrogate` paints the same
 window with the trained network (quietmap_amd/surrogate_params.npy), `loudness` runs
 zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
-`--aircraft-scale`, and `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
+`--aircraft-scale` (`passes` with its 48 pass classes per period), `pixels` runs the pixel
+loudness kernels (mixtures_kernel, ndens_kernel) over `--pixels` synthetic packed pixels, and
+`copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
 line when it ends.
 """
@@ -31,6 +33,7 @@ def main():
     ap.add_argument("--cells", type=int, default=16 * 1024 * 1024)
     ap.add_argument("--aircraft-scale", type=float, default=0.05)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--pixels", type=int, default=4096)
     args = ap.parse_args()
     kernels = args.kernels.split(",")
     gpu = Gpu(0)
@@ -83,6 +86,20 @@ def main():
         r = gpu.aircraft_grid(ac)
         print(f"aircraft: {ac.grid_shape} receivers, {ac.tracks.size} tracks, events binned "
               f"{int(r['hist'].sum())}, {time.time() - t:.2f} s", flush=True)
+    if "passes" in kernels:
+        ac = scenes.aircraft_scene(scale=args.aircraft_scale)
+        t = time.time()
+        r = gpu.aircraft_grid(ac, passes=True)
+        print(f"passes: {ac.grid_shape} receivers, {ac.tracks.size} tracks, classes held "
+              f"{int((r['passes'][:, 1] > 0).sum())}, {time.time() - t:.2f} s", flush=True)
+    if "pixels" in kernels:
+        from . import pixels
+
+        packed, offsets = pixels.concat(pixels.synthetic_pixels(seed=21, count=args.pixels))
+        t = time.time()
+        nd = gpu.pixel_loudness(packed, offsets)
+        print(f"pixels: {args.pixels} pixels, place Nden mean {float(nd[:, 6].mean()):.2f} sone, "
+              f"{time.time() - t:.2f} s", flush=True)
     if "copy" in kernels:
         nbytes = args.copy_mib << 20
         a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)

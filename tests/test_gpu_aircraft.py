@@ -42,3 +42,24 @@ def test_aircraft_grid_matches_oracle(gpu, scale):
           f"N65 total {int(want['n65'].sum())}", flush=True)
     assert err.max() <= 0.05
     assert want["hist"].sum() > 100
+
+
+def test_pass_classes_match_oracle(gpu):
+    """The 48 pass classes per period: every heard segment's pass in the oracle's class (the
+    passing shares, FP64 geometry, agree to 1e-9 and sit in the same classes), each class's energy
+    (FP32 SEL energies summed in FP64) within the Lden bar of 0.05 dB."""
+    print("case: aircraft pass classes scale 0.02", flush=True)
+    sc = scenes.aircraft_scene(scale=0.02)
+    got = gpu.aircraft_grid(sc, passes=True)
+    rng = np.random.default_rng(2)
+    n = sc.points_xy.shape[0]
+    sample = rng.choice(n, min(200, n), replace=False)
+    want = oracle.aircraft_points(sc, sc.points_xy[sample], passes=True)
+    g, w = got["passes"][sample], want["passes"]
+    assert np.array_equal(g[:, 1] > 0, w[:, 1] > 0)
+    assert np.allclose(g[:, 1], w[:, 1], rtol=1e-9, atol=0)
+    held = w[:, 0] > 0
+    err = np.abs(10 * np.log10(g[:, 0][held] / w[:, 0][held]))
+    print(f"pass classes: {int(held.sum())} classes held, energy max |err| {err.max():.5f} dB", flush=True)
+    assert err.max() <= 0.05
+    assert np.array_equal(got["hist"][sample], want["hist"])

@@ -66,7 +66,10 @@ def test_outputs_match_the_oracle(gpu):
     want, _ = mlp.forward(params.astype(np.float64), x, rounding=True)
     err = np.max(np.abs(y - want)) / np.max(np.abs(want))
     print(f"outputs: relative max error {err:.2e}", flush=True)
-    assert err < 1e-3
+    # FP32 accumulation in another order flips BF16 roundings of near-tie activations: the FP64
+    # restatement itself moves by 0.4e-3 to 1.5e-3 of the output scale when its weights are
+    # perturbed by 2^-22 relative (six draws), so the bar is twice the worst of those.
+    assert err < 3e-3
     net.close()
 
 

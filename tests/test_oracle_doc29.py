@@ -90,3 +90,77 @@ def test_histogram_counts_flights():
     assert np.all(r["hist"].sum(axis=1) <= flights)
     assert np.all(r["n70"] <= r["n65"])
     assert r["segments"].sum() > 0 and np.all(r["period_energy"] >= 0)
+
+
+SECONDS = np.array([12.0, 4.0, 8.0]) * 3600.0
+
+
+def heard_lengths(sc, x, y):
+    """Per track, the summed 3-D length of its segments within reach of (x, y), by brute force."""
+    v = sc.vertices.reshape(-1, 3).astype(np.float64)
+    out = np.zeros(sc.tracks.size)
+    for t, tr in enumerate(sc.tracks):
+        p = v[tr["first_vertex"]:tr["first_vertex"] + tr["vertex_count"]]
+        a, b = p[:-1], p[1:]
+        s = a[:, :2] - [x, y]
+        d = b[:, :2] - a[:, :2]
+        ll = np.sum(d * d, 1)
+        u = np.clip(-np.sum(s * d, 1) / np.where(ll > 0, ll, 1.0), 0.0, 1.0) * (ll > 0)
+        q = s + u[:, None] * d
+        heard = np.sum(q * q, 1) <= sc.reach_m ** 2
+        out[t] = np.sqrt(np.sum((b - a) ** 2, 1))[heard].sum()
+    return out
+
+
+def test_pass_classes_hold_the_energy_and_the_time_in_progress():
+    """The classes' energies sum to the period's SEL energy over its seconds, and their passing
+    shares to every heard segment's flights times its duration at the track's ground speed
+    (popup/src/aircraft/boxes.rs:225-265 with the segment as the chord)."""
+    sc = scenes.aircraft_scene(scale=0.02)
+    pts = sc.points_xy[::37][:12]
+    r = oracle.aircraft_points(sc, pts, passes=True)
+    energy, passing = r["passes"][:, 0], r["passes"][:, 1]
+    want = r["period_energy"] / SECONDS
+    assert np.allclose(energy.sum(axis=2), want, rtol=1e-12, atol=0)
+    v_ref = sc.classes["v_ref_kt"][sc.tracks["aircraft_class"]]
+    speed = np.where(sc.tracks["speed_kt"] > 10.0, sc.tracks["speed_kt"].astype(np.float64), v_ref)
+    spm = 3600.0 / (speed * 1852.0)
+    flights = sc.tracks["flights"].astype(np.float64)
+    for i, (x, y) in enumerate(pts):
+        lengths = heard_lengths(sc, x, y)
+        expected = (flights * (lengths * spm)[:, None]).sum(axis=0) / SECONDS
+        assert np.allclose(passing[i].sum(axis=1), expected, rtol=1e-9, atol=0), i
+    assert (passing > 0).sum() > 50
+
+
+def test_a_pass_lands_in_the_class_of_its_level():
+    """One level segment: its pass sits in the class of 10 lg(energy / passing) (3 dB classes from
+    -30 dB, aircraft.rs:86-95), the same class in every period with flights, clamped at both ends."""
+    sc = scenes.aircraft_scene(scale=0.01)
+    tr = sc.tracks[:1].copy()
+    tr["first_vertex"], tr["vertex_count"] = 0, 2
+    tr["flights"] = [3, 0, 2]
+    cases = ((300.0, 2000.0, 0.0), (3000.0, 800.0, 0.0), (60.0, 20000.0, 0.0), (9000.0, 5.0, 0.0),
+             (30000.0, 1000.0, 0.0), (5.0, -999.9, -1000.0))
+    hit = set()
+    for height, x1, rx in cases:
+        verts = np.array([[-1000.0, 50.0, height], [x1, 50.0, height]], np.float32)
+        one = scenes.AircraftScene(np.array([[rx, 50.0]]), (1, 1), sc.classes, tr, verts, 0.0, 1e9)
+        r = oracle.aircraft_points(one, one.points_xy, passes=True)
+        rel = verts.astype(np.float64) - [rx, 50.0, 0.0]
+        sel, _ = oracle.segment_levels(sc.classes, tr[0], rel[0], rel[1])
+        v = float(sc.classes[tr[0]["aircraft_class"]]["v_ref_kt"])
+        speed = float(tr[0]["speed_kt"]) if tr[0]["speed_kt"] > 10 else v
+        duration = float(np.linalg.norm(verts[1].astype(np.float64) - verts[0])) * 3600.0 / (speed * 1852.0)
+        level = sel - 10 * math.log10(duration)
+        k = min(max(int(math.floor((level + 30.0) / 3.0)), 0), abi.PASS_CLASSES - 1)
+        hit.add(k)
+        for p, f in enumerate(tr[0]["flights"]):
+            row = r["passes"][0, 1, p]
+            if f == 0:
+                assert not row.any()
+                continue
+            assert np.flatnonzero(row).tolist() == [k], (height, p)
+            assert abs(r["passes"][0, 0, p, k] - f * 10 ** (sel / 10) / SECONDS[p]) <= 1e-9 * r["passes"][0, 0, p, k]
+            assert abs(row[k] - f * duration / SECONDS[p]) <= 1e-12 * row[k]
+    assert {0, abi.PASS_CLASSES - 1} <= hit
