@@ -99,11 +99,16 @@ int qm_loudness_device(qm_gpu* gpu, const float* d_levels, size_t n, size_t cell
  * FlightPasses::add_pass (popup/src/aircraft.rs:86-95, the card's pass_class and flight_sums
  * class sums, paint-gpu/kernels/aircraft.cuh:250-311), each heard segment a pass at its SEL energy
  * over the period's seconds while its flights are on it (box_passing, boxes.rs:225-265, with the
- * segment's 3-D length as the chord). Host arrays. */
+ * segment's 3-D length as the chord). With `screening` (null: unscreened), each segment's SEL
+ * below the ceiling is screened by the receiver's horizons (segment.rs:284-350 over
+ * screening.rs and horizons.rs; the reference card's horizon_loss_db, aircraft.cuh:83-100): the
+ * terrain's loss at the line's closest point, the buildings' at the segment's, their larger beyond
+ * the lateral attenuation; LAmax is not screened. Host arrays. */
 int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_classes, const qm_track* tracks,
                      size_t n_tracks, const float* vertices, size_t n_vertices, const double* points_xy, uint32_t nx,
                      uint32_t ny, double receiver_altitude_m, double reach_m, double* period_energy, uint32_t* n65,
-                     uint32_t* n70, uint32_t* hist, double* passes, char* err, size_t errlen);
+                     uint32_t* n70, uint32_t* hist, double* passes, const qm_aircraft_screening* screening,
+                     char* err, size_t errlen);
 
 /* Pixel loudness on the card (the reference's Gpu::pixel_loudness, paint-gpu/src/loudness.rs, over
  * its kernels pixel_mixtures and pixel_ndens, paint-gpu/kernels/distribution.cuh and

@@ -247,8 +247,56 @@ Closest closest_points(const double s[3], const double e[3]) {
 
 static double norm3(const double p[3]) { return std::hypot(std::hypot(p[0], p[1]), p[2]); }
 
-// segment.rs:318-350 screened_segment_sel, unscreened.
-double segment_sel_db(const Emission& em, const Closest& c) {
+// screening.rs:19-22 path_difference_loss_db
+double path_difference_loss_db(double path_difference_m) {
+    const double raw = 10.0 * std::log10(3.0 + QM_PATH_DIFFERENCE_SLOPE_PER_M * std::max(path_difference_m, 0.0));
+    return std::min(std::max(raw - QM_GRAZING_LOSS_DB, 0.0), QM_SCREENING_CAP_DB);
+}
+
+// screening.rs:29-44 edge_loss_db
+double edge_loss_db(double edge_range_m, double edge_height_m, double source_range_m, double source_height_m) {
+    const bool blocks = edge_range_m < source_range_m && edge_height_m * source_range_m > source_height_m * edge_range_m;
+    if (!blocks) return 0.0;
+    const double receiver_to_edge = std::hypot(edge_range_m, edge_height_m);
+    const double edge_to_source = std::hypot(source_range_m - edge_range_m, source_height_m - edge_height_m);
+    const double receiver_to_source = std::hypot(source_range_m, source_height_m);
+    return path_difference_loss_db(receiver_to_edge + edge_to_source - receiver_to_source);
+}
+
+// horizons.rs:107-111 sector_of
+int sector_of(double east, double north, int sectors) {
+    const double tau = 2.0 * M_PI;
+    double angle = std::fmod(std::atan2(north, east), tau);
+    if (angle < 0.0) angle += tau;
+    return std::min((int)(angle / tau * (double)sectors), sectors - 1);
+}
+
+// horizons.rs:84-104 SectorEdges::loss_db over the card's floats (write_f32): the largest edge loss
+// toward `point` (east, north, up from the receiver) over its sector's edges.
+double horizon_loss_db(const float* at, int sectors, const double point[3]) {
+    const double range_m = std::hypot(point[0], point[1]);
+    if (range_m <= 1.0 || point[2] >= (double)at[0] * range_m) return 0.0;
+    const int sector = sector_of(point[0], point[1], sectors);
+    const float* edges = at + 1 + sector * (1 + 2 * QM_HORIZON_BANDS);
+    if (point[2] >= (double)edges[0] * range_m) return 0.0;
+    double loss = 0.0;
+    for (int band = 0; band < QM_HORIZON_BANDS; ++band) {
+        const double edge_range = edges[1 + 2 * band], tangent = edges[2 + 2 * band];
+        if (std::isnan(edge_range)) continue;
+        loss = std::max(loss, edge_loss_db(edge_range, tangent * edge_range, range_m, point[2]));
+    }
+    return loss;
+}
+
+// segment.rs:284-350 segment_sel_at_receiver: free SEL less the screening's excess over the lateral
+// attenuation (screening.rs:49-56), the terrain horizon asked at the closest point of the line, the
+// buildings' at the closest point of the segment, none at or above the ceiling.
+struct Screening {
+    const float* horizon;  // QM_HORIZON_FLOATS, or null
+    double height_above_ground_m;
+};
+
+double segment_sel_db(const Emission& em, const Closest& c, const Screening& scr) {
     const double slant = norm3(c.on_line);
     const Reading npd = em.read_npd(slant);
     const double finite = 10.0 * std::log10(finite_segment_energy_fraction(c.along * c.length_m, c.length_m,
@@ -256,7 +304,12 @@ double segment_sel_db(const Emission& em, const Closest& c) {
     const double lateral = lateral_attenuation_db(c.lambda_height_m, c.lateral_m);
     const double installation = 10.0 * std::log10(installation_energy_gain(em.installation, c.height_m, slant));
     const double free_sel = npd.sel_db + em.speed_correction_db + installation - lateral + finite;
-    return free_sel - std::max(0.0 - lateral, 0.0);  // screening.rs:56-63 with no losses
+    double terrain = 0.0, buildings = 0.0;
+    if (scr.horizon && scr.height_above_ground_m < QM_SCREENING_CEILING_M) {
+        terrain = horizon_loss_db(scr.horizon, QM_TERRAIN_SECTORS, c.on_line);
+        buildings = horizon_loss_db(scr.horizon + QM_TERRAIN_HORIZON_FLOATS, QM_BUILDING_SECTORS, c.on_segment);
+    }
+    return free_sel - std::max(std::max(terrain, buildings) - lateral, 0.0);
 }
 
 // segment.rs:357-381 segment_lmax_db (receiver impedance zero at the datum).
@@ -343,7 +396,7 @@ int qmo_segment_levels(const qm_aircraft_class* classes, const qm_track* track, 
     Emission em{&cls, (int)track->departure, (int)track->power_row, (double)track->power_weight,
                 speed_correction_db(src.v_ref_kt, (double)track->speed_kt), src.installation};
     const Closest c = closest_points(s, e);
-    *sel_db = segment_sel_db(em, c);
+    *sel_db = segment_sel_db(em, c, Screening{nullptr, 0.0});
     *lmax_db = segment_lmax_db(em, c);
     return 0;
 }
@@ -354,7 +407,8 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
                         const float* vertices, const double* points /* x, y pairs */, size_t n_points,
                         double receiver_altitude_m, double reach_m, double* energy_out /* 3 a point */,
                         uint32_t* n65, uint32_t* n70, uint32_t* hist /* QM_HIST_BINS a point */,
-                        double* passes /* QM_PASS_FLOATS a point, or null */, uint64_t* segments_heard,
+                        double* passes /* QM_PASS_FLOATS a point, or null */,
+                        const qm_aircraft_screening* screening /* or null */, uint64_t* segments_heard,
                         int threads) {
     std::vector<Class> cls(n_classes);
     for (size_t k = 0; k < n_classes; ++k) {
@@ -381,6 +435,9 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
         double* class_passing = passes ? class_energy + 3 * QM_PASS_CLASSES : nullptr;
         if (passes)
             for (int k = 0; k < QM_PASS_FLOATS; ++k) class_energy[k] = 0.0;
+        const float* horizon = nullptr;
+        if (screening && screening->horizons && screening->horizon_of && screening->horizon_of[i] != QM_NO_HORIZON)
+            horizon = screening->horizons + (size_t)screening->horizon_of[i] * QM_HORIZON_FLOATS;
         for (size_t t = 0; t < n_tracks; ++t) {
             const qm_track& tr = tracks[t];
             const Class& c = cls[tr.aircraft_class];
@@ -398,7 +455,11 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
                 any = true;
                 ++heard;
                 const Closest cp = closest_points(s, en);
-                const double sel_energy = energy(segment_sel_db(em, cp));
+                const size_t va = tr.first_vertex + v;
+                const double ga = (screening && screening->vertex_ground_m) ? screening->vertex_ground_m[va] : 0.0;
+                const double gb = (screening && screening->vertex_ground_m) ? screening->vertex_ground_m[va + 1] : 0.0;
+                const Screening scr{horizon, 0.5 * ((double)a[2] - ga + (double)b[2] - gb)};
+                const double sel_energy = energy(segment_sel_db(em, cp, scr));
                 track_energy += sel_energy;
                 lmax = std::max(lmax, segment_lmax_db(em, cp));
                 if (passes) {
@@ -433,6 +494,14 @@ int qmo_aircraft_points(const qm_aircraft_class* classes, size_t n_classes, cons
         if (segments_heard) segments_heard[i] = heard;
     });
     return 0;
+}
+
+double qmo_path_difference_loss(double d) { return path_difference_loss_db(d); }
+double qmo_edge_loss(double er, double eh, double sr, double sh) { return edge_loss_db(er, eh, sr, sh); }
+int qmo_sector_of(double east, double north, int sectors) { return sector_of(east, north, sectors); }
+double qmo_horizon_loss(const float* at, int sectors, double east, double north, double up) {
+    const double p[3] = {east, north, up};
+    return horizon_loss_db(at, sectors, p);
 }
 
 }  // extern "C"

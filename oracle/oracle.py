@@ -96,7 +96,15 @@ def _declare(L):
     L.qmo_closest_points.argtypes = [D, D, D]
     L.qmo_segment_levels.argtypes = [C.c_void_p, C.c_void_p, D, D, D, D]
     L.qmo_aircraft_points.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, D, C.c_size_t,
-                                      C.c_double, C.c_double, D, U32, U32, U32, D, U64, C.c_int]
+                                      C.c_double, C.c_double, D, U32, U32, U32, D, C.c_void_p, U64, C.c_int]
+    L.qmo_path_difference_loss.argtypes = [C.c_double]
+    L.qmo_path_difference_loss.restype = C.c_double
+    L.qmo_edge_loss.argtypes = [C.c_double] * 4
+    L.qmo_edge_loss.restype = C.c_double
+    L.qmo_sector_of.argtypes = [C.c_double, C.c_double, C.c_int]
+    L.qmo_sector_of.restype = C.c_int
+    L.qmo_horizon_loss.argtypes = [F32, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.qmo_horizon_loss.restype = C.c_double
 
 
 def finite_fraction(along, length, scaled):
@@ -131,9 +139,27 @@ def segment_levels(classes, track, start, end):
     return sel[0], lmax[0]
 
 
-def aircraft_points(scene, points_xy, threads=None, passes=False):
+class Screening(C.Structure):
+    _fields_ = [("horizons", C.c_void_p), ("n_horizons", C.c_size_t), ("horizon_of", C.c_void_p),
+                ("vertex_ground_m", C.c_void_p)]
+
+
+def screening_struct(screening):
+    """A qm_aircraft_screening over a dict of `horizons` (n, QM_HORIZON_FLOATS) float32,
+    `horizon_of` (per receiver, uint32) and optional `vertex_ground_m` (float32); the arrays it
+    points at are returned too, to be kept alive."""
+    h = np.ascontiguousarray(screening["horizons"], np.float32)
+    of = np.ascontiguousarray(screening["horizon_of"], np.uint32)
+    g = screening.get("vertex_ground_m")
+    g = None if g is None else np.ascontiguousarray(g, np.float32)
+    st = Screening(h.ctypes.data, h.shape[0], of.ctypes.data, None if g is None else g.ctypes.data)
+    return st, (h, of, g)
+
+
+def aircraft_points(scene, points_xy, threads=None, passes=False, screening=None):
     """Per receiver: SEL energy per period, N65, N70, LAmax histogram, segments heard, and with
-    `passes` the pass classes [n, 2, 3, 48] (energies, then passing shares)."""
+    `passes` the pass classes [n, 2, 3, 48] (energies, then passing shares). `screening`: a dict
+    (`screening_struct`) whose `horizon_of` is indexed by these points."""
     xy = np.ascontiguousarray(points_xy, np.float64).reshape(-1, 2)
     n = xy.shape[0]
     energy = np.zeros((n, 3))
@@ -141,13 +167,16 @@ def aircraft_points(scene, points_xy, threads=None, passes=False):
     hist = np.zeros((n, abi.HIST_BINS), np.uint32)
     heard = np.zeros(n, np.uint64)
     pc = np.zeros((n, 2, 3, abi.PASS_CLASSES)) if passes else None
+    if screening:
+        st, keep = screening_struct(screening)
     classes = np.ascontiguousarray(scene.classes, abi.AIRCRAFT_CLASS_DTYPE)
     tracks = np.ascontiguousarray(scene.tracks, abi.TRACK_DTYPE)
     verts = np.ascontiguousarray(scene.vertices, np.float32)
     rc = lib().qmo_aircraft_points(classes.ctypes.data, classes.size, tracks.ctypes.data, tracks.size,
                                    verts.ctypes.data, _p(xy, D), n, scene.receiver_altitude_m, scene.reach_m,
                                    _p(energy, D), _p(n65, U32), _p(n70, U32), _p(hist, U32),
-                                   _p(pc, D) if passes else None, _p(heard, U64), threads or os.cpu_count() or 1)
+                                   _p(pc, D) if passes else None, C.byref(st) if screening else None,
+                                   _p(heard, U64), threads or os.cpu_count() or 1)
     if rc != 0:
         raise ValueError("oracle: track out of its class's power rows")
     out = dict(period_energy=energy, n65=n65, n70=n70, hist=hist, segments=heard)
@@ -451,3 +480,20 @@ def pixel_loudness(packed, offsets, bin_db=2.0, mixtures=False, threads=None):
     lib().qmo_pixel_loudness(_p(packed, F32), _p(offsets, U64), n, float(bin_db), _p(mix, D) if mixtures else None,
                              _p(nd, D), threads or os.cpu_count() or 1)
     return (nd, mix) if mixtures else nd
+
+
+def path_difference_loss(d):
+    return lib().qmo_path_difference_loss(float(d))
+
+
+def edge_loss(edge_range, edge_height, source_range, source_height):
+    return lib().qmo_edge_loss(float(edge_range), float(edge_height), float(source_range), float(source_height))
+
+
+def sector_of(east, north, sectors):
+    return lib().qmo_sector_of(float(east), float(north), int(sectors))
+
+
+def horizon_loss(record, sectors, east, north, up):
+    r = np.ascontiguousarray(record, np.float32)
+    return lib().qmo_horizon_loss(_p(r, F32), int(sectors), float(east), float(north), float(up))

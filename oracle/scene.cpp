@@ -30,11 +30,9 @@ static bool node_at(const qm_terrain& T, int64_t row, int64_t column, Node& out)
 
 static bool node_valid(const Node& n) { return n.height_code != 0xffffu && n.impervious <= 100u; }
 
-// tiles/src/terrain.rs:342-385 Terrain::sample_at, the fractions measured north and east from
-// the cell's south-west node.
-int ground_at(const qm_terrain& T, double x, double y, GroundSample& out) {
-    const double cp = (x - T.origin_x_m) / T.spacing_m;
-    const double rp = (y - T.origin_y_m) / T.spacing_m;
+// tiles/src/terrain.rs:342-385 Terrain::sample_at at lattice position (cp, rp), the fractions
+// measured north and east from the cell's south-west node.
+static int cell_sample(const qm_terrain& T, double cp, double rp, GroundSample& out) {
     const double column = std::floor(cp), row = std::floor(rp);
     const double cf = cp - column, rf = rp - row;
     const int64_t c = (int64_t)column, r = (int64_t)row;
@@ -57,6 +55,80 @@ int ground_at(const qm_terrain& T, double x, double y, GroundSample& out) {
     out.ground_factor = 1.0 - impervious / 100.0;
     out.forest_cover = (double)nearest.forest / 100.0;
     out.canopy_m = (double)nearest.canopy;
+    return 0;
+}
+
+// Fine terrain (tiles/src/terrain_fine.rs): detail in steps of QM_DETAIL_STEP_M on the lattice
+// QM_FINE_PER_NODE times as fine, read where the block of QM_FINE_BLOCK_CELLS square cells holding
+// the point has fine terrain (the reference's tiles with a fine_offset).
+static uint32_t fine_rows(const qm_terrain& T) { return QM_FINE_PER_NODE * (T.rows - 1) + 1; }
+static uint32_t fine_columns(const qm_terrain& T) { return QM_FINE_PER_NODE * (T.columns - 1) + 1; }
+
+static bool fine_block(const qm_terrain& T, double cp, double rp) {
+    if (!T.fine_detail || !T.fine_blocks) return false;
+    const int64_t bx_n = ((int64_t)T.columns - 2) / QM_FINE_BLOCK_CELLS + 1;
+    const int64_t by_n = ((int64_t)T.rows - 2) / QM_FINE_BLOCK_CELLS + 1;
+    const int64_t bx = std::min(std::max((int64_t)std::floor(cp / QM_FINE_BLOCK_CELLS), (int64_t)0), bx_n - 1);
+    const int64_t by = std::min(std::max((int64_t)std::floor(rp / QM_FINE_BLOCK_CELLS), (int64_t)0), by_n - 1);
+    return T.fine_blocks[by * bx_n + bx] != 0;
+}
+
+// paint-gpu/kernels/terrain.cuh:91-103 fine_detail_at (tiles/src/terrain_fine.rs): the detail's
+// bilinear value at fine position (row, column).
+static double detail_at(const qm_terrain& T, double row, double column) {
+    const uint32_t rows = fine_rows(T), columns = fine_columns(T);
+    row = std::max(row, 0.0);
+    column = std::max(column, 0.0);
+    const uint32_t r = std::min((uint32_t)std::floor(row), rows - 2), c = std::min((uint32_t)std::floor(column), columns - 2);
+    const double fr = row - r, fc = column - c;
+    const int16_t* d = T.fine_detail;
+    const size_t at = (size_t)r * columns + c;
+    const double sum = (1.0 - fr) * ((1.0 - fc) * d[at] + fc * d[at + 1]) +
+                       fr * ((1.0 - fc) * d[at + columns] + fc * d[at + columns + 1]);
+    return sum * QM_DETAIL_STEP_M;
+}
+
+// popup/src/scene.rs Ground::at: the 1/6" height where the point's block has fine terrain.
+int ground_at(const qm_terrain& T, double x, double y, GroundSample& out) {
+    const double cp = (x - T.origin_x_m) / T.spacing_m;
+    const double rp = (y - T.origin_y_m) / T.spacing_m;
+    const int failed = cell_sample(T, cp, rp, out);
+    if (failed) return failed;
+    if (fine_block(T, cp, rp)) out.height_m += detail_at(T, QM_FINE_PER_NODE * rp, QM_FINE_PER_NODE * cp);
+    return 0;
+}
+
+// popup/src/scene.rs Ground::fine_sample (paint-gpu/kernels/terrain_walk.cuh:105-160): a vertex of
+// the fine walk on fine row line `line` (on_row) or fine column line `line` between fine rows
+// `south` and south + 1, `f` along the line: the coarse cell's sample at the vertex's place (its
+// fine indices over QM_FINE_PER_NODE) plus, in a block with fine terrain, the detail between the
+// line's two fine nodes.
+static int fine_sample(const qm_terrain& T, bool on_row, int64_t line, int64_t south, double position, double f,
+                       GroundSample& out) {
+    const int64_t rows = fine_rows(T), columns = fine_columns(T);
+    int64_t r0, c0;
+    if (on_row) {
+        r0 = line;
+        c0 = (int64_t)std::floor(position);
+    } else {
+        r0 = south;
+        c0 = line;
+    }
+    const int64_t r1 = on_row ? r0 : r0 + 1, c1 = on_row ? c0 + 1 : c0;
+    if (r0 < 0 || c0 < 0 || r0 >= rows || c0 >= columns || (f > 0.0 && (r1 >= rows || c1 >= columns)))
+        return QM_FAILED_NO_TERRAIN;
+    const double rp = (on_row ? (double)r0 : (double)r0 + f) / QM_FINE_PER_NODE;
+    const double cp = (on_row ? (double)c0 + f : (double)c0) / QM_FINE_PER_NODE;
+    const int failed = cell_sample(T, cp, rp, out);
+    if (failed) return failed;
+    if (fine_block(T, cp, rp)) {
+        const int16_t* d = T.fine_detail;
+        double d0 = f < 1.0 ? (double)d[r0 * columns + c0] : 0.0;
+        double d1 = f > 0.0 ? (double)d[r1 * columns + c1] : 0.0;
+        if (f <= 0.0) d1 = d0;
+        if (f >= 1.0) d0 = d1;
+        out.height_m += (d0 + f * (d1 - d0)) * QM_DETAIL_STEP_M;
+    }
     return 0;
 }
 
@@ -88,12 +160,15 @@ static void push_sample(Profile& profile, double t, const GroundSample& s) {
     profile.push(t, s.height_m, s.ground_factor, s.forest_cover, s.canopy_m);
 }
 
-// popup/src/scene.rs:333-420 Ground::walk on the planar lattice: a vertex wherever the ray
-// a -> b crosses a row or column line strictly between fractions `from` and `to`, read between
-// that line's two nodes; at a node (both lines at once) the row line's reading, both advanced.
-static int walk(const qm_terrain& T, const double a[2], const double b[2], double from, double to, Profile& profile) {
-    const double ca = (a[0] - T.origin_x_m) / T.spacing_m, cb = (b[0] - T.origin_x_m) / T.spacing_m;
-    const double ra = (a[1] - T.origin_y_m) / T.spacing_m, rb = (b[1] - T.origin_y_m) / T.spacing_m;
+// popup/src/scene.rs:333-420 Ground::walk on the planar lattice `scale` times as fine as the
+// coarse one: a vertex wherever the ray a -> b crosses a row or column line strictly between
+// fractions `from` and `to`, read between that line's two nodes (the fine sample on the fine
+// lattice); at a node (both lines at once) the row line's reading, both advanced.
+static int walk(const qm_terrain& T, int scale, const double a[2], const double b[2], double from, double to,
+                Profile& profile) {
+    const double unit = T.spacing_m / scale;
+    const double ca = (a[0] - T.origin_x_m) / unit, cb = (b[0] - T.origin_x_m) / unit;
+    const double ra = (a[1] - T.origin_y_m) / unit, rb = (b[1] - T.origin_y_m) / unit;
     const double columns = cb - ca, rise = rb - ra;
     const double column_from = ca + from * columns, row_from = ra + from * rise;
     double next_column, column_step;
@@ -123,23 +198,33 @@ static int walk(const qm_terrain& T, const double a[2], const double b[2], doubl
         const double t_row = rise != 0.0 ? (next_row - ra) / rise : inf;
         const double t = std::min(t_column, t_row);
         if (t >= to) break;
-        Node n0{}, n1{};
-        bool h0, h1;
-        double f;
-        if (t_row <= t_column) {
+        GroundSample sample;
+        int failed;
+        if (scale == 1) {
+            Node n0{}, n1{};
+            bool h0, h1;
+            double f;
+            if (t_row <= t_column) {
+                const double c = ca + t * columns;
+                const double west = std::floor(c);
+                f = c - west;
+                h0 = node_at(T, (int64_t)next_row, (int64_t)west, n0);
+                h1 = node_at(T, (int64_t)next_row, (int64_t)west + 1, n1);
+            } else {
+                const double r = ra + t * rise;
+                f = std::min(std::max(r - south, 0.0), 1.0);
+                h0 = node_at(T, (int64_t)south, (int64_t)next_column, n0);
+                h1 = node_at(T, (int64_t)south + 1, (int64_t)next_column, n1);
+            }
+            failed = edge_sample(h0, n0, h1, n1, f, sample);
+        } else if (t_row <= t_column) {
             const double c = ca + t * columns;
-            const double west = std::floor(c);
-            f = c - west;
-            h0 = node_at(T, (int64_t)next_row, (int64_t)west, n0);
-            h1 = node_at(T, (int64_t)next_row, (int64_t)west + 1, n1);
+            failed = fine_sample(T, true, (int64_t)next_row, 0, c, c - std::floor(c), sample);
         } else {
             const double r = ra + t * rise;
-            f = std::min(std::max(r - south, 0.0), 1.0);
-            h0 = node_at(T, (int64_t)south, (int64_t)next_column, n0);
-            h1 = node_at(T, (int64_t)south + 1, (int64_t)next_column, n1);
+            failed = fine_sample(T, false, (int64_t)next_column, (int64_t)south, r,
+                                 std::min(std::max(r - south, 0.0), 1.0), sample);
         }
-        GroundSample sample;
-        const int failed = edge_sample(h0, n0, h1, n1, f, sample);
         if (failed) return failed;
         push_sample(profile, t, sample);
         if (t_row <= t_column) {
@@ -156,8 +241,21 @@ static int walk(const qm_terrain& T, const double a[2], const double b[2], doubl
     return 0;
 }
 
-// popup/src/scene.rs:242-309 Ground::fill_profile (one lattice: the synthetic scene has no fine
-// terrain, the reference's `fine_root` None branch).
+// popup/src/scene.rs Ground::fine_near: whether a block under the stretch a -> b (its box's
+// corners) has fine terrain.
+static bool fine_near(const qm_terrain& T, const double a[2], const double b[2]) {
+    if (!T.fine_detail) return false;
+    const double xs[2] = {std::min(a[0], b[0]), std::max(a[0], b[0])};
+    const double ys[2] = {std::min(a[1], b[1]), std::max(a[1], b[1])};
+    for (double x : xs)
+        for (double y : ys)
+            if (fine_block(T, (x - T.origin_x_m) / T.spacing_m, (y - T.origin_y_m) / T.spacing_m)) return true;
+    return false;
+}
+
+// popup/src/scene.rs:242-309 Ground::fill_profile: the 1/6" lattice within QM_FINE_ENDS_M of either
+// end where a block under that stretch has fine terrain, the coarse lattice elsewhere, neighbours
+// on one lattice joined, a vertex (Ground::at) where the lattice changes.
 int fill_profile(const qm_terrain& T, const double source[2], const double receiver[2], Profile& profile) {
     const double offset[2] = {source[0] - receiver[0], source[1] - receiver[1]};
     const double length = std::hypot(offset[0], offset[1]);
@@ -166,8 +264,45 @@ int fill_profile(const qm_terrain& T, const double source[2], const double recei
     int failed = ground_at(T, source[0], source[1], start);
     if (failed) return failed;
     push_sample(profile, 0.0, start);
-    failed = walk(T, source, receiver, 0.0, 1.0, profile);
-    if (failed) return failed;
+    auto along = [&](double t, double out[2]) {
+        out[0] = source[0] + t * (receiver[0] - source[0]);
+        out[1] = source[1] + t * (receiver[1] - source[1]);
+    };
+    struct Stretch {
+        double from, to;
+        int scale;
+    };
+    Stretch stretches[3] = {{0.0, 1.0, 1}, {0.0, 1.0, 1}, {0.0, 1.0, 1}};
+    int count = 1;
+    if (!T.fine_detail) {
+        count = 1;
+    } else if (length <= 2.0 * QM_FINE_ENDS_M) {
+        stretches[0].scale = fine_near(T, source, receiver) ? QM_FINE_PER_NODE : 1;
+    } else {
+        const double e = QM_FINE_ENDS_M / length;
+        double pe[2], pf[2];
+        along(e, pe);
+        along(1.0 - e, pf);
+        const Stretch candidates[3] = {{0.0, e, fine_near(T, source, pe) ? QM_FINE_PER_NODE : 1},
+                                       {e, 1.0 - e, 1},
+                                       {1.0 - e, 1.0, fine_near(T, pf, receiver) ? QM_FINE_PER_NODE : 1}};
+        count = 0;
+        for (const Stretch& c : candidates) {
+            if (count > 0 && stretches[count - 1].scale == c.scale) stretches[count - 1].to = c.to;
+            else stretches[count++] = c;
+        }
+    }
+    for (int k = 0; k < count; ++k) {
+        failed = walk(T, stretches[k].scale, source, receiver, stretches[k].from, stretches[k].to, profile);
+        if (failed) return failed;
+        if (stretches[k].to < 1.0) {
+            const double to = stretches[k].to;
+            GroundSample sw;
+            failed = ground_at(T, source[0] - to * offset[0], source[1] - to * offset[1], sw);
+            if (failed) return failed;
+            push_sample(profile, to, sw);
+        }
+    }
     failed = ground_at(T, receiver[0], receiver[1], end);
     if (failed) return failed;
     push_sample(profile, 1.0, end);

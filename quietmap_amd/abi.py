@@ -79,7 +79,15 @@ class CTerrain(C.Structure):
         ("rows", C.c_uint32),
         ("nodes", C.c_void_p),
         ("canopy_m", C.c_void_p),
+        ("fine_detail", C.c_void_p),
+        ("fine_blocks", C.c_void_p),
     ]
+
+
+FINE_PER_NODE = 6
+DETAIL_STEP_M = 0.2
+FINE_ENDS_M = 100.0
+FINE_BLOCK_CELLS = 32
 
 
 class CObstacles(C.Structure):
@@ -115,15 +123,24 @@ class CWeather(C.Structure):
 
 class Terrain:
     """A terrain lattice: ``nodes`` (rows, columns) u32 = height code | imperviousness << 16 |
-    forest << 24, ``canopy_m`` (rows, columns) u8; row r at y = origin_y + r*spacing."""
+    forest << 24, ``canopy_m`` (rows, columns) u8; row r at y = origin_y + r*spacing. Optional
+    fine terrain: ``fine_detail`` (FINE_PER_NODE*(rows-1)+1, FINE_PER_NODE*(columns-1)+1) i16 steps
+    of DETAIL_STEP_M and ``fine_blocks`` (block rows, block columns) u8 flags of FINE_BLOCK_CELLS
+    square blocks."""
 
-    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, canopy_m=None):
+    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, canopy_m=None, fine_detail=None, fine_blocks=None):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         rows, columns = self.nodes.shape
         self.canopy_m = (
             np.zeros((rows, columns), np.uint8) if canopy_m is None else np.ascontiguousarray(canopy_m, np.uint8)
         )
         self.origin_x_m, self.origin_y_m, self.spacing_m = float(origin_x_m), float(origin_y_m), float(spacing_m)
+        self.fine_detail = self.fine_blocks = None
+        if fine_detail is not None:
+            self.fine_detail = np.ascontiguousarray(fine_detail, np.int16)
+            assert self.fine_detail.shape == (FINE_PER_NODE * (rows - 1) + 1, FINE_PER_NODE * (columns - 1) + 1)
+            self.fine_blocks = np.ascontiguousarray(fine_blocks, np.uint8)
+            assert self.fine_blocks.shape == ((rows - 2) // FINE_BLOCK_CELLS + 1, (columns - 2) // FINE_BLOCK_CELLS + 1)
         self.c = CTerrain(
             self.origin_x_m,
             self.origin_y_m,
@@ -132,6 +149,8 @@ class Terrain:
             rows,
             self.nodes.ctypes.data,
             self.canopy_m.ctypes.data,
+            None if self.fine_detail is None else self.fine_detail.ctypes.data,
+            None if self.fine_blocks is None else self.fine_blocks.ctypes.data,
         )
 
     @staticmethod
@@ -211,6 +230,18 @@ PASS_CLASSES = 48
 PASS_CLASS_DB = 3.0
 PASS_LOWEST_DB = -30.0
 PASS_FLOATS = 2 * 3 * PASS_CLASSES
+HORIZON_BANDS = 6
+TERRAIN_SECTORS = 32
+BUILDING_SECTORS = 256
+TERRAIN_HORIZON_FLOATS = 1 + TERRAIN_SECTORS * (1 + 2 * HORIZON_BANDS)
+HORIZON_FLOATS = TERRAIN_HORIZON_FLOATS + 1 + BUILDING_SECTORS * (1 + 2 * HORIZON_BANDS)
+NO_HORIZON = 0xFFFFFFFF
+
+
+class AircraftScreening(C.Structure):
+    """qm_aircraft_screening (csrc/qm_aircraft.h)."""
+    _fields_ = [("horizons", C.c_void_p), ("n_horizons", C.c_size_t), ("horizon_of", C.c_void_p),
+                ("vertex_ground_m", C.c_void_p)]
 AIRCRAFT_CLASS_DTYPE = np.dtype(
     [
         ("sel_db", "<f8", (2, POWER_ROWS, NPD_DISTANCES)),

@@ -82,6 +82,9 @@ struct AircraftArgs {
     uint32_t* n70;
     uint32_t* hist;
     double* passes;  // QM_PASS_FLOATS a receiver, or null
+    const float* horizons;          // QM_HORIZON_FLOATS a record, or null
+    const uint32_t* horizon_of;     // a record per receiver (QM_NO_HORIZON: unscreened)
+    const float* vertex_ground;     // terrain under each vertex, or null (0)
     double seconds[3];
     uint32_t* counters;
     Tables tables;
@@ -256,6 +259,37 @@ __device__ double finite_fraction_d(double along, double length, double scaled) 
     return fmax((e1 - e0) * (1.0 / PI_D), 1e-15);
 }
 
+// screening.rs:19-44 and horizons.rs:84-111 in FP64, in the oracle's operation order.
+__device__ double path_difference_loss_d(double delta) {
+#pragma clang fp contract(off)
+    const double raw = 10.0 * log10(3.0 + QM_PATH_DIFFERENCE_SLOPE_PER_M * fmax(delta, 0.0));
+    return fmin(fmax(raw - QM_GRAZING_LOSS_DB, 0.0), QM_SCREENING_CAP_DB);
+}
+
+__device__ double edge_loss_d(double er, double eh, double sr, double sh) {
+#pragma clang fp contract(off)
+    if (!(er < sr && eh * sr > sh * er)) return 0.0;
+    return path_difference_loss_d(hypot(er, eh) + hypot(sr - er, sh - eh) - hypot(sr, sh));
+}
+
+__device__ double horizon_loss_d(const float* at, int sectors, const double* p) {
+#pragma clang fp contract(off)
+    const double range = hypot(p[0], p[1]);
+    if (range <= 1.0 || p[2] >= (double)at[0] * range) return 0.0;
+    double angle = fmod(atan2(p[1], p[0]), 2.0 * PI_D);
+    if (angle < 0.0) angle += 2.0 * PI_D;
+    const int sector = min((int)(angle / (2.0 * PI_D) * (double)sectors), sectors - 1);
+    const float* edges = at + 1 + sector * (1 + 2 * QM_HORIZON_BANDS);
+    if (p[2] >= (double)edges[0] * range) return 0.0;
+    double loss = 0.0;
+    for (int band = 0; band < QM_HORIZON_BANDS; ++band) {
+        const double r = edges[1 + 2 * band], t = edges[2 + 2 * band];
+        if (isnan(r)) continue;
+        loss = fmax(loss, edge_loss_d(r, t * r, range, p[2]));
+    }
+    return loss;
+}
+
 // One power row's SEL and scaled distance at an NPD position (npd.rs:290-330 PowerRow::read).
 __device__ void row_read_d(const DevRow& r, int interval, double fraction, double slant, double& sel,
                            double& scaled) {
@@ -277,31 +311,32 @@ __device__ void row_read_d(const DevRow& r, int interval, double fraction, doubl
 // The SEL (dB) of segment (a, b) in FP64, in the oracle's operation order (segment.rs:318-350 over
 // the closest points, SegmentEmission::read_npd and corrections.rs), unscreened.
 __device__ __noinline__ double segment_sel_d(const AircraftArgs& g, const DevTrack& tr, const DevRow* rows, const float* a,
-                                const float* b, double rx, double ry) {
+                                const float* b, double rx, double ry, const float* horizon) {
 #pragma clang fp contract(off)
     const double s[3] = {(double)a[0] - rx, (double)a[1] - ry, (double)a[2] - g.receiver_altitude_m};
     const double e[3] = {(double)b[0] - rx, (double)b[1] - ry, (double)b[2] - g.receiver_altitude_m};
     const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
     const double hs = d[0] * d[0] + d[1] * d[1];
     const double ls = hs + d[2] * d[2];
-    double along, on_line[3], lateral, height, lambda_height;
+    double along, on_line[3], on_seg[3], lateral, height, lambda_height;
     if (hs <= 1e-6) {
         along = 0.0;
-        for (int k = 0; k < 3; ++k) on_line[k] = s[k];
+        for (int k = 0; k < 3; ++k) on_line[k] = on_seg[k] = s[k];
         lateral = hypot(s[0], s[1]);
         height = s[2];
         lambda_height = s[2];
     } else {
         along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / ls;
         const double cl = fmin(fmax(along, 0.0), 1.0);
-        double on_seg2 = 0.0;
-        for (int k = 0; k < 3; ++k) on_line[k] = s[k] + along * d[k];
-        on_seg2 = s[2] + cl * d[2];
+        for (int k = 0; k < 3; ++k) {
+            on_line[k] = s[k] + along * d[k];
+            on_seg[k] = s[k] + cl * d[k];
+        }
         lateral = fabs(s[0] * d[1] - s[1] * d[0]) / sqrt(hs);
         const double ss = on_line[0] * on_line[0] + on_line[1] * on_line[1] + on_line[2] * on_line[2];
         height = copysign(sqrt(fmax(ss - lateral * lateral, 0.0)), on_line[2]);
         const double cos_climb = sqrt(hs / ls);
-        lambda_height = (along >= 0.0 && along <= 1.0) ? height : on_seg2 / cos_climb;
+        lambda_height = (along >= 0.0 && along <= 1.0) ? height : on_seg[2] / cos_climb;
     }
     const double length = fmax(sqrt(ls), 1.0);
     const double slant_in = hypot(hypot(on_line[0], on_line[1]), on_line[2]);
@@ -330,7 +365,12 @@ __device__ __noinline__ double segment_sel_d(const AircraftArgs& g, const DevTra
     const double lat = lateral_d(lambda_height, lateral);
     const double installation = 10.0 * log10(installation_d(tr.installation, height, slant_in));
     const double free_sel = sel + tr.dv_d + installation - lat + finite;
-    return free_sel - fmax(0.0 - lat, 0.0);
+    double terrain = 0.0, buildings = 0.0;
+    if (horizon) {
+        terrain = horizon_loss_d(horizon, QM_TERRAIN_SECTORS, on_line);
+        buildings = horizon_loss_d(horizon + QM_TERRAIN_HORIZON_FLOATS, QM_BUILDING_SECTORS, on_seg);
+    }
+    return free_sel - fmax(fmax(terrain, buildings) - lat, 0.0);
 }
 
 // FlightPasses::add_pass's class in FP64 (popup/src/aircraft.rs:86-95).
@@ -354,15 +394,51 @@ __device__ bool within_reach_d(const float* a, const float* b, double rx, double
     return px * px + py * py <= reach * reach;
 }
 
+// screening.rs:19-44 in FP32: the single-edge path difference |RE| + |ES| - |RS| as
+// 2 (u x v)^2 / ((|u||v| + u.v)(|u| + |v| + |u + v|)), u = RE, v = ES, which keeps its digits
+// where the direct sum of three kilometre lengths loses the metre-scale difference.
+__device__ __forceinline__ float path_difference_loss_f(float delta) {
+    const float raw = 10.0f * log10f(3.0f + (float)QM_PATH_DIFFERENCE_SLOPE_PER_M * fmaxf(delta, 0.0f));
+    return fminf(fmaxf(raw - (float)QM_GRAZING_LOSS_DB, 0.0f), (float)QM_SCREENING_CAP_DB);
+}
+
+__device__ __forceinline__ float edge_loss_f(float er, float eh, float sr, float sh) {
+    if (!(er < sr && eh * sr > sh * er)) return 0.0f;
+    const float vx = sr - er, vy = sh - eh;
+    const float a = sqrtf(er * er + eh * eh), b = sqrtf(vx * vx + vy * vy), c = sqrtf(sr * sr + sh * sh);
+    const float cross = er * vy - eh * vx, dot = er * vx + eh * vy;
+    return path_difference_loss_f(2.0f * cross * cross / ((a * b + dot) * (a + b + c)));
+}
+
+// horizons.rs:84-104 SectorEdges::loss_db in FP32 over the card's floats.
+__device__ float horizon_loss_f(const float* at, int sectors, float x, float y, float z) {
+    const float range = sqrtf(x * x + y * y);
+    if (range <= 1.0f || z >= at[0] * range) return 0.0f;
+    float angle = atan2f(y, x);
+    if (angle < 0.0f) angle += (float)(2.0 * PI_D);
+    const int sector = min((int)(angle * (float)(sectors / (2.0 * PI_D))), sectors - 1);
+    const float* edges = at + 1 + sector * (1 + 2 * QM_HORIZON_BANDS);
+    if (z >= edges[0] * range) return 0.0f;
+    float loss = 0.0f;
+    for (int band = 0; band < QM_HORIZON_BANDS; ++band) {
+        const float r = edges[1 + 2 * band];
+        if (isnan(r)) continue;
+        loss = fmaxf(loss, edge_loss_f(r, edges[2 + 2 * band] * r, range, z));
+    }
+    return loss;
+}
+
 struct SegmentF {
     float sel_energy;  // 10^(SEL/10), 0 when not heard
     float lmax;        // -inf when not heard
 };
 
 // One segment's SEL energy and LAmax in FP32.
+// `horizon`: the receiver's horizons when the segment is screened (its mean height above the
+// ground under its ends below QM_SCREENING_CEILING_M), else null.
 __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTrack& tr, const DevRow* rows,
                                              const float* a, const float* b, float rxf, float ryf, double rx,
-                                             double ry) {
+                                             double ry, const float* horizon) {
     SegmentF out{0.0f, -INFINITY};
     const float s[3] = {a[0] - rxf, a[1] - ryf, a[2] - (float)g.receiver_altitude_m};
     const float e[3] = {b[0] - rxf, b[1] - ryf, b[2] - (float)g.receiver_altitude_m};
@@ -432,7 +508,16 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     const float lat = lateral_f(lambda_height, lateral);
     const float inst = installation_f(tr.installation, height, slant);
     // 10^(SEL/10) with SEL = NPD + Delta_V + Delta_I - Lambda + Delta_F (segment.rs:318-350).
-    out.sel_energy = exp10f((sel + tr.dv - lat) * 0.1f) * inst * finite;
+    float screening = 0.0f;
+    if (horizon) {
+        // segment.rs:296-313: the terrain horizon at the line's closest point, the buildings' at
+        // the segment's; their larger loss beyond the lateral attenuation (screening.rs:49-56).
+        const float terrain = horizon_loss_f(horizon, QM_TERRAIN_SECTORS, on_line[0], on_line[1], on_line[2]);
+        const float buildings = horizon_loss_f(horizon + QM_TERRAIN_HORIZON_FLOATS, QM_BUILDING_SECTORS, on_seg[0],
+                                               on_seg[1], on_seg[2]);
+        screening = fmaxf(fmaxf(terrain, buildings) - lat, 0.0f);
+    }
+    out.sel_energy = exp10f((sel + tr.dv - lat - screening) * 0.1f) * inst * finite;
     // LAmax (segment.rs:357-381).
     const float distance = sqrtf(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1] + on_seg[2] * on_seg[2]);
     float lamax;
@@ -460,9 +545,10 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
 
 }  // namespace
 
-// PASSES: the pass classes are summed (g.passes is not null); the kernel without them keeps the
-// registers of the energies, counts and histogram alone.
-template <bool PASSES>
+// PASSES: the pass classes are summed (g.passes is not null); SCREEN: the receivers' horizons
+// screen the segments (g.horizons is not null). The kernel without either keeps the registers of
+// the energies, counts and histogram alone.
+template <bool PASSES, bool SCREEN>
 __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
     __shared__ uint32_t s_tile;
     const int lane = threadIdx.x;
@@ -484,6 +570,9 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
         uint32_t* hist = g.hist + index * QM_HIST_BINS;
         if (valid)
             for (int k = 0; k < QM_HIST_BINS; ++k) hist[k] = 0u;
+        const float* horizon = nullptr;
+        if (SCREEN && valid && g.horizon_of[index] != QM_NO_HORIZON)
+            horizon = g.horizons + (size_t)g.horizon_of[index] * QM_HORIZON_FLOATS;
         // Pass classes: a run of one class per period sums in registers and is added to the
         // receiver's row when the class changes.
         double* pass_row = (PASSES && valid) ? g.passes + index * QM_PASS_FLOATS : nullptr;
@@ -517,7 +606,16 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
             if (valid) {
                 const float* v = g.vertices + 3 * (size_t)tr.first_vertex;
                 for (uint32_t k = 0; k + 1 < tr.vertex_count; ++k) {
-                    const SegmentF sf = segment_f(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rxf, ryf, rx, ry);
+                    // Screened below the ceiling over the ground under its ends (segment.rs:296-313).
+                    const float* screen = nullptr;
+                    if (SCREEN && horizon) {
+                        const size_t va = (size_t)tr.first_vertex + k;
+                        const double ga = g.vertex_ground ? (double)g.vertex_ground[va] : 0.0;
+                        const double gb = g.vertex_ground ? (double)g.vertex_ground[va + 1] : 0.0;
+                        const double above = 0.5 * ((double)v[3 * k + 2] - ga + (double)v[3 * k + 5] - gb);
+                        if (above < QM_SCREENING_CEILING_M) screen = horizon;
+                    }
+                    const SegmentF sf = segment_f(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rxf, ryf, rx, ry, screen);
                     if (sf.lmax == -INFINITY) continue;
                     any = true;
                     e_track += sf.sel_energy;
@@ -535,7 +633,8 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
                         const bool near_edge =
                             edge >= 1.0f && edge <= (float)(QM_PASS_CLASSES - 1) &&
                             fabsf(level - ((float)QM_PASS_LOWEST_DB + (float)QM_PASS_CLASS_DB * edge)) < EPS_DB;
-                        const double exact = near_edge ? energy_d(segment_sel_d(g, tr, g.rows, pa, pa + 3, rx, ry)) : 0.0;
+                        const double exact =
+                            near_edge ? energy_d(segment_sel_d(g, tr, g.rows, pa, pa + 3, rx, ry, screen)) : 0.0;
                         for (int p = 0; p < 3; ++p) {
                             const double seconds = g.seconds[p];
                             const double e_p = (double)tr.flights[p] * (double)sf.sel_energy / seconds;
@@ -570,7 +669,7 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
             if (m3 >= m1 - EPS_DB) {
                 // Three or more within reach of the maximum: every such segment again.
                 for (uint32_t k = 0; k + 1 < tr.vertex_count; ++k) {
-                    const SegmentF sf = segment_f(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rxf, ryf, rx, ry);
+                    const SegmentF sf = segment_f(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rxf, ryf, rx, ry, nullptr);
                     if (sf.lmax >= m1 - EPS_DB)
                         lmax = fmax(lmax, segment_lmax_d(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rx, ry));
                 }
@@ -672,7 +771,8 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
                                 size_t n_tracks, const float* vertices, size_t n_vertices, const double* points_xy,
                                 uint32_t nx, uint32_t ny, double receiver_altitude_m, double reach_m,
                                 double* period_energy, uint32_t* n65, uint32_t* n70, uint32_t* hist,
-                                double* passes, char* err, size_t errlen) {
+                                double* passes, const qm_aircraft_screening* screening, char* err,
+                                size_t errlen) {
     if (!gpu || (n_tracks && (!classes || !tracks || !vertices)) || !points_xy || !period_energy || !n65 || !n70 || !hist)
         return afail(err, errlen, "null argument");
     const size_t n = (size_t)nx * ny;
@@ -772,6 +872,14 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
     }
     for (size_t i = 0; i < 2 * n; ++i)
         if (!std::isfinite(points_xy[i])) return afail(err, errlen, "point %zu: not finite", i / 2);
+    const bool screen = screening && screening->horizons;
+    if (screen) {
+        if (!screening->horizon_of) return afail(err, errlen, "screening: horizons without horizon_of");
+        for (size_t i = 0; i < n; ++i)
+            if (screening->horizon_of[i] != QM_NO_HORIZON && screening->horizon_of[i] >= screening->n_horizons)
+                return afail(err, errlen, "receiver %zu: horizon %u of %zu", i, screening->horizon_of[i],
+                             screening->n_horizons);
+    }
     A_TRY(hipSetDevice(gpu->ordinal));
     AircraftArgs g;
     std::memset(&g, 0, sizeof g);
@@ -780,9 +888,11 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
         g.tables.log_npd[k] = (float)g.tables.log_npd_d[k];
     }
     void *d_rows = nullptr, *d_tracks = nullptr, *d_vertices = nullptr, *d_points = nullptr, *d_energy = nullptr,
-         *d_n65 = nullptr, *d_n70 = nullptr, *d_hist = nullptr, *d_counters = nullptr, *d_passes = nullptr;
+         *d_n65 = nullptr, *d_n70 = nullptr, *d_hist = nullptr, *d_counters = nullptr, *d_passes = nullptr,
+         *d_horizons = nullptr, *d_horizon_of = nullptr, *d_ground = nullptr;
     auto cleanup = [&]() {
-        for (void* p : {d_rows, d_tracks, d_vertices, d_points, d_energy, d_n65, d_n70, d_hist, d_counters, d_passes})
+        for (void* p : {d_rows, d_tracks, d_vertices, d_points, d_energy, d_n65, d_n70, d_hist, d_counters, d_passes,
+                        d_horizons, d_horizon_of, d_ground})
             if (p) (void)hipFree(p);
     };
     hipError_t e = hipSuccess;
@@ -799,6 +909,18 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
     alloc(&d_hist, sizeof(uint32_t) * QM_HIST_BINS * n);
     alloc(&d_counters, sizeof(uint32_t) * qm::XCDS);
     if (passes) alloc(&d_passes, sizeof(double) * QM_PASS_FLOATS * n);
+    if (screen) {
+        alloc(&d_horizons, sizeof(float) * QM_HORIZON_FLOATS * screening->n_horizons);
+        alloc(&d_horizon_of, sizeof(uint32_t) * n);
+        if (screening->vertex_ground_m) alloc(&d_ground, sizeof(float) * n_vertices);
+        if (e == hipSuccess && screening->n_horizons)
+            e = hipMemcpy(d_horizons, screening->horizons, sizeof(float) * QM_HORIZON_FLOATS * screening->n_horizons,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d_horizon_of, screening->horizon_of, sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && d_ground && n_vertices)
+            e = hipMemcpy(d_ground, screening->vertex_ground_m, sizeof(float) * n_vertices, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && !rows.empty()) e = hipMemcpy(d_rows, rows.data(), sizeof(DevRow) * rows.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && n_tracks) e = hipMemcpy(d_tracks, dev_tracks.data(), sizeof(DevTrack) * n_tracks, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_vertices) e = hipMemcpy(d_vertices, vertices, sizeof(float) * 3 * n_vertices, hipMemcpyHostToDevice);
@@ -823,20 +945,29 @@ extern "C" int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, s
     g.n70 = (uint32_t*)d_n70;
     g.hist = (uint32_t*)d_hist;
     g.passes = (double*)d_passes;
+    g.horizons = (const float*)d_horizons;
+    g.horizon_of = (const uint32_t*)d_horizon_of;
+    g.vertex_ground = (const float*)d_ground;
     for (int p = 0; p < 3; ++p) g.seconds[p] = QM_PERIOD_HOURS[p] * 3600.0;
     g.counters = (uint32_t*)d_counters;
     hipLaunchKernelGGL(qm::zero_u32_kernel, dim3(1), dim3(64), 0, 0, g.counters, (uint32_t)qm::XCDS);
     int cus = 0, per_cu = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal);
-    const void* kernel = passes ? reinterpret_cast<const void*>(aircraft_kernel<true>)
-                                : reinterpret_cast<const void*>(aircraft_kernel<false>);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+    const void* kernels[2][2] = {{reinterpret_cast<const void*>(aircraft_kernel<false, false>),
+                                  reinterpret_cast<const void*>(aircraft_kernel<false, true>)},
+                                 {reinterpret_cast<const void*>(aircraft_kernel<true, false>),
+                                  reinterpret_cast<const void*>(aircraft_kernel<true, true>)}};
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[passes != nullptr][screen], 256, 0);
     unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
     groups = std::min<unsigned>(groups, g.n_tiles);
-    if (passes)
-        hipLaunchKernelGGL(aircraft_kernel<true>, dim3(groups), dim3(256), 0, 0, g);
+    if (passes && screen)
+        hipLaunchKernelGGL((aircraft_kernel<true, true>), dim3(groups), dim3(256), 0, 0, g);
+    else if (passes)
+        hipLaunchKernelGGL((aircraft_kernel<true, false>), dim3(groups), dim3(256), 0, 0, g);
+    else if (screen)
+        hipLaunchKernelGGL((aircraft_kernel<false, true>), dim3(groups), dim3(256), 0, 0, g);
     else
-        hipLaunchKernelGGL(aircraft_kernel<false>, dim3(groups), dim3(256), 0, 0, g);
+        hipLaunchKernelGGL((aircraft_kernel<false, false>), dim3(groups), dim3(256), 0, 0, g);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(period_energy, d_energy, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(n65, d_n65, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);

@@ -11,6 +11,7 @@
 #ifndef QM_AIRCRAFT_H
 #define QM_AIRCRAFT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #define QM_NPD_DISTANCES 10
@@ -36,6 +37,24 @@
 static const double QM_PERIOD_HOURS[3] = {12.0, 4.0, 8.0};
 #endif
 
+/* Horizon screening (physics/src/doc29/screening.rs, popup/src/aircraft/horizons.rs): a
+ * receiver's horizons as the reference writes them for its card (Horizons::write_f32): the
+ * terrain's, then the buildings'; each the steepest tangent of all, then per sector (anticlockwise
+ * from east) its steepest tangent and per range band the steepest edge's range (m) and tangent,
+ * NaN range for an empty band. */
+#define QM_HORIZON_BANDS 6
+#define QM_TERRAIN_SECTORS 32
+#define QM_BUILDING_SECTORS 256
+#define QM_TERRAIN_HORIZON_FLOATS (1 + QM_TERRAIN_SECTORS * (1 + 2 * QM_HORIZON_BANDS))
+#define QM_HORIZON_FLOATS (QM_TERRAIN_HORIZON_FLOATS + 1 + QM_BUILDING_SECTORS * (1 + 2 * QM_HORIZON_BANDS))
+#define QM_NO_HORIZON 0xffffffffu
+/* 10 lg 3 removed so the loss is 0 at grazing; 20 / 0.685 m a metre of path difference; the cap;
+ * the height above the ground from which a segment is not screened. */
+#define QM_GRAZING_LOSS_DB 4.771212547196624
+#define QM_PATH_DIFFERENCE_SLOPE_PER_M 29.2
+#define QM_SCREENING_CAP_DB 18.0
+#define QM_SCREENING_CEILING_M 7620.0
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -47,6 +66,18 @@ typedef struct qm_aircraft_class {
     uint32_t installation;                              /* QM_INSTALL_* */
     double v_ref_kt;
 } qm_aircraft_class;
+
+/* Horizons for the aircraft grid: `horizons` holds `n_horizons` records of QM_HORIZON_FLOATS;
+ * receiver i reads record horizon_of[i] (QM_NO_HORIZON: unscreened). `vertex_ground_m`, when not
+ * null, is the terrain height under each track vertex (same datum as the vertices), else 0: a
+ * segment at least QM_SCREENING_CEILING_M above the ground under its ends (their mean) is not
+ * screened. */
+typedef struct qm_aircraft_screening {
+    const float* horizons;
+    size_t n_horizons;
+    const uint32_t* horizon_of;
+    const float* vertex_ground_m;
+} qm_aircraft_screening;
 
 typedef struct qm_track {
     uint32_t first_vertex; /* into the vertex array: x, y, z float triples (frame metres, z above datum) */

@@ -50,12 +50,26 @@
 extern "C" {
 #endif
 
+/* Fine terrain (tiles/src/terrain_fine.rs): the lattice QM_FINE_PER_NODE times as fine, detail
+ * over the coarse bilinear ground in steps of QM_DETAIL_STEP_M, read within QM_FINE_ENDS_M of a
+ * ray's ends (physics/src/profile.rs:13) where a block of QM_FINE_BLOCK_CELLS square coarse cells
+ * under that stretch has fine terrain (the reference's tiles with a fine_offset). */
+#define QM_FINE_PER_NODE 6
+#define QM_DETAIL_STEP_M 0.2
+#define QM_FINE_ENDS_M 100.0
+#define QM_FINE_BLOCK_CELLS 32
+
 typedef struct qm_terrain {
     double origin_x_m, origin_y_m;
     double spacing_m;
     uint32_t columns, rows;
     const uint32_t* nodes;   /* rows*columns: height code | impervious << 16 | forest << 24 */
     const uint8_t* canopy_m; /* rows*columns */
+    /* Optional fine terrain (null: none): detail steps on (QM_FINE_PER_NODE (rows-1) + 1) x
+     * (QM_FINE_PER_NODE (columns-1) + 1) fine nodes, row-major from the south-west node, and a
+     * flag per block ((columns-2)/QM_FINE_BLOCK_CELLS + 1 blocks a row, likewise for rows). */
+    const int16_t* fine_detail;
+    const uint8_t* fine_blocks;
 } qm_terrain;
 
 typedef struct qm_run {

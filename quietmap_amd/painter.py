@@ -86,8 +86,8 @@ def library():
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
         lib.qm_evaluate_line_pairs.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, sz, vp, vp, vp, vp, cp, sz]
-        lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, vp, cp,
-                                         sz]
+        lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, vp, vp,
+                                         cp, sz]
         lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
         lib.qm_kurze_table.argtypes = [vp]
         lib.qm_kurze_table.restype = None
@@ -236,10 +236,12 @@ class Gpu:
                                           stream or None, err, 512)
         _check(rc, err)
 
-    def aircraft_grid(self, scene, passes=False):
+    def aircraft_grid(self, scene, passes=False, screening=None):
         """Doc 29 aircraft outputs of an AircraftScene's receiver grid: SEL energy per period,
-        N65, N70 and the LAmax histogram per receiver, and with `passes` the pass classes
-        [n, 2, 3, 48] (energies, then passing shares, per period and class)."""
+        N65, N70 and the LAmax histogram per receiver, with `passes` the pass classes
+        [n, 2, 3, 48] (energies, then passing shares, per period and class), and with `screening`
+        (a dict of `horizons` [records, QM_HORIZON_FLOATS] float32, `horizon_of` per receiver and
+        optional `vertex_ground_m` per vertex) the SEL screened by the receivers' horizons."""
         nx, ny = scene.grid_shape
         xy = np.ascontiguousarray(scene.points_xy, np.float64).reshape(-1, 2)
         assert xy.shape[0] == nx * ny
@@ -251,11 +253,21 @@ class Gpu:
         n65, n70 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
         hist = np.zeros((n, abi.HIST_BINS), np.uint32)
         pc = np.zeros((n, 2, 3, abi.PASS_CLASSES)) if passes else None
+        scr = None
+        if screening is not None:
+            h = np.ascontiguousarray(screening["horizons"], np.float32).reshape(-1, abi.HORIZON_FLOATS)
+            of = np.ascontiguousarray(screening["horizon_of"], np.uint32)
+            assert of.size == n
+            gr = screening.get("vertex_ground_m")
+            gr = None if gr is None else np.ascontiguousarray(gr, np.float32)
+            assert gr is None or gr.size == verts.shape[0]
+            scr = abi.AircraftScreening(h.ctypes.data, h.shape[0], of.ctypes.data, None if gr is None else gr.ctypes.data)
         err = C.create_string_buffer(512)
         rc = library().qm_aircraft_grid(self.handle, _ptr(classes), classes.size, _ptr(tracks), tracks.size,
                                         _ptr(verts), verts.shape[0], _ptr(xy), nx, ny, scene.receiver_altitude_m,
                                         scene.reach_m, _ptr(energy), _ptr(n65), _ptr(n70), _ptr(hist),
-                                        _ptr(pc) if passes else None, err, 512)
+                                        _ptr(pc) if passes else None, C.byref(scr) if scr is not None else None,
+                                        err, 512)
         _check(rc, err)
         out = dict(period_energy=energy, n65=n65, n70=n70, hist=hist)
         if passes:

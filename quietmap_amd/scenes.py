@@ -479,3 +479,68 @@ def line_scene(seed=5, receivers=96, spacing=10.0, lines=30, buildings=300):
     extent = receivers * spacing
     sc.pieces = line_pieces(rng, lines, (extent, extent), (0.0, 0.0))
     return sc
+
+
+TERRAIN_BAND_LIMITS_M = (500.0, 1000.0, 2000.0, 3500.0, 5500.0, 8000.0)
+BUILDING_BAND_LIMITS_M = (16.0, 32.0, 64.0, 128.0, 256.0, 512.0)
+
+
+def _sector_edges(rng, sectors, limits, tangent_lo, tangent_hi, empty):
+    """One SectorEdges as the card reads it: steepest of all, then per sector its steepest and its
+    bands' (range, tangent), NaN range for an empty band."""
+    out = [0.0]
+    steepest_all = -np.inf
+    lower = (0.0,) + limits[:-1]
+    for _ in range(sectors):
+        bands, steepest = [], -np.inf
+        for lo, hi in zip(lower, limits):
+            if rng.uniform() < empty:
+                bands += [np.nan, np.nan]
+                continue
+            r = rng.uniform(max(lo, 1.0), hi)
+            t = rng.uniform(tangent_lo, tangent_hi)
+            bands += [r, t]
+            steepest = max(steepest, t)
+        out += [steepest] + bands
+        steepest_all = max(steepest_all, steepest)
+    out[0] = steepest_all
+    return out
+
+
+def aircraft_horizons(scene, seed=9, block=8):
+    """Synthetic horizons for the aircraft grid: one record a `block` x `block` receivers (terrain
+    ridges to a tangent of 0.06 within 8 km, buildings to a tangent of 1.2 within 512 m, a tenth of
+    the blocks unscreened), and the terrain under every vertex (0 to 150 m). A dict for
+    qm_aircraft_screening."""
+    rng = np.random.default_rng(seed)
+    nx, ny = scene.grid_shape
+    bx, by = (nx + block - 1) // block, (ny + block - 1) // block
+    records = []
+    for _ in range(bx * by):
+        rec = _sector_edges(rng, abi.TERRAIN_SECTORS, TERRAIN_BAND_LIMITS_M, -0.02, 0.06, 0.2)
+        rec += _sector_edges(rng, abi.BUILDING_SECTORS, BUILDING_BAND_LIMITS_M, 0.05, 1.2, 0.6)
+        records.append(rec)
+    horizons = np.array(records, np.float32)
+    assert horizons.shape[1] == abi.HORIZON_FLOATS
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    of = ((jj // block) * bx + ii // block).reshape(-1).astype(np.uint32)
+    of[rng.uniform(size=of.size) < 0.1] = abi.NO_HORIZON
+    ground = rng.uniform(0.0, 150.0, scene.vertices.reshape(-1, 3).shape[0]).astype(np.float32)
+    return dict(horizons=horizons, horizon_of=of, vertex_ground_m=ground)
+
+
+def with_fine_terrain(terrain, seed=13, share=0.5, amplitude_m=4.0):
+    """`terrain` (abi.Terrain) with fine terrain: seeded detail of up to `amplitude_m` (smooth
+    ripples plus noise, in QM_DETAIL_STEP_M steps) on the lattice six times as fine, read in
+    `share` of its 32-cell blocks."""
+    rng = np.random.default_rng(seed)
+    rows, columns = terrain.nodes.shape
+    fr, fc = abi.FINE_PER_NODE * (rows - 1) + 1, abi.FINE_PER_NODE * (columns - 1) + 1
+    yy, xx = np.mgrid[0:fr, 0:fc].astype(np.float64)
+    ripple = np.sin(xx / 7.3 + rng.uniform(0, 6)) * np.cos(yy / 5.1 + rng.uniform(0, 6))
+    detail_m = amplitude_m * (0.7 * ripple + 0.3 * rng.uniform(-1, 1, (fr, fc)))
+    detail = np.round(detail_m / abi.DETAIL_STEP_M).astype(np.int16)
+    blocks = (rng.uniform(size=((rows - 2) // abi.FINE_BLOCK_CELLS + 1, (columns - 2) // abi.FINE_BLOCK_CELLS + 1))
+              < share).astype(np.uint8)
+    return abi.Terrain(terrain.origin_x_m, terrain.origin_y_m, terrain.spacing_m, terrain.nodes, terrain.canopy_m,
+                       fine_detail=detail, fine_blocks=blocks)

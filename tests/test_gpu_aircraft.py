@@ -63,3 +63,37 @@ def test_pass_classes_match_oracle(gpu):
     print(f"pass classes: {int(held.sum())} classes held, energy max |err| {err.max():.5f} dB", flush=True)
     assert err.max() <= 0.05
     assert np.array_equal(got["hist"][sample], want["hist"])
+
+
+def test_screened_grid_matches_oracle(gpu):
+    """Horizon screening (terrain and building horizons, ground under the vertices): Lden within
+    0.05 dB, N65/N70 and the histogram unchanged and bit-exact, pass classes in the oracle's
+    classes; the screening lowers some receivers' energy."""
+    print("case: aircraft screened scale 0.03", flush=True)
+    sc = scenes.aircraft_scene(scale=0.03)
+    scr = scenes.aircraft_horizons(sc)
+    got = gpu.aircraft_grid(sc, passes=True, screening=scr)
+    free = gpu.aircraft_grid(sc)
+    rng = np.random.default_rng(4)
+    n = sc.points_xy.shape[0]
+    sample = rng.choice(n, min(300, n), replace=False)
+    want = oracle.aircraft_points(sc, sc.points_xy[sample], passes=True,
+                                  screening=dict(scr, horizon_of=scr["horizon_of"][sample]))
+    for k in ("n65", "n70", "hist"):
+        assert np.array_equal(got[k][sample], want[k])
+        assert np.array_equal(got[k], free[k])
+    from quietmap_amd.painter import lden_db
+
+    hours = np.array([12.0, 4.0, 8.0]) * 3600.0
+    g = lden_db(got["period_energy"][sample] / hours)
+    w = lden_db(want["period_energy"] / hours)
+    ok = np.isfinite(w)
+    assert np.array_equal(np.isfinite(g), ok)
+    err = np.abs(g[ok] - w[ok])
+    drop = lden_db(free["period_energy"][sample] / hours)[ok] - w[ok]
+    print(f"screened Lden: n={ok.sum()} max |err| {err.max():.5f} dB; screening up to {drop.max():.2f} dB", flush=True)
+    assert err.max() <= 0.05
+    assert drop.max() > 0.5
+    gp, wp = got["passes"][sample], want["passes"]
+    assert np.array_equal(gp[:, 1] > 0, wp[:, 1] > 0)
+    assert np.allclose(gp[:, 1], wp[:, 1], rtol=1e-9, atol=0)

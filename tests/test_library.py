@@ -36,3 +36,12 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(painter, "_lib", None)
     with pytest.raises(painter.PainterError):
         painter.library()
+
+
+def test_screening_struct_matches_the_header():
+    import ctypes as C
+
+    from quietmap_amd import abi
+
+    assert C.sizeof(abi.AircraftScreening) == 32
+    assert abi.HORIZON_FLOATS == 3746 and abi.TERRAIN_HORIZON_FLOATS == 417

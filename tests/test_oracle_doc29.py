@@ -164,3 +164,60 @@ def test_a_pass_lands_in_the_class_of_its_level():
             assert abs(r["passes"][0, 0, p, k] - f * 10 ** (sel / 10) / SECONDS[p]) <= 1e-9 * r["passes"][0, 0, p, k]
             assert abs(row[k] - f * duration / SECONDS[p]) <= 1e-12 * row[k]
     assert {0, abi.PASS_CLASSES - 1} <= hit
+
+
+def test_screening_loss_and_its_composition():
+    """screening_tests.rs: zero at grazing, capped at 18 dB; an edge screens only between receiver
+    and source above the line of sight; smooth across the shadow boundary."""
+    assert oracle.path_difference_loss(0.0) == 0.0 and oracle.path_difference_loss(-1.0) == 0.0
+    six = 10.0 * math.log10((3.0 + 29.2 * 6.0) / 3.0)
+    assert abs(oracle.path_difference_loss(6.0) - six) < 1e-12
+    assert oracle.path_difference_loss(6.5) == 18.0
+    assert oracle.edge_loss(500.0, 500.0, 3000.0, 0.0) == 18.0
+    assert oracle.edge_loss(1000.0, 100.0, 2000.0, 150.0) > 5.0
+    for args in ((1000.0, 100.0, 800.0, 20.0), (1000.0, 100.0, 1000.0, 20.0), (1000.0, 100.0, 2000.0, 250.0),
+                 (1000.0, 100.0, 2000.0, 200.0), (1000.0, 100.0, 0.0, 500.0)):
+        assert oracle.edge_loss(*args) == 0.0
+    losses = [oracle.edge_loss(158.0, 20.0, 5000.0, 5000.0 * math.tan(math.radians(6.7 + 0.1 * k))) for k in range(11)]
+    assert any(x > 0 for x in losses) and 0.0 in losses
+    assert all(abs(b - a) < 0.5 for a, b in zip(losses, losses[1:]))
+
+
+def test_horizons_screen_what_lies_behind_their_edges():
+    """horizons_tests.rs: sectors by direction; a ridge 40 m high 1 km east (the steeper of two edges
+    in its band) screens a low aircraft 5 km east, not one west, nor one before the ridge, nor one
+    above its line of sight."""
+    assert oracle.sector_of(1.0, 0.001, 32) == 0 and oracle.sector_of(0.0, 1.0, 32) == 8
+    assert oracle.sector_of(-1.0, -0.001, 32) == 16
+    assert oracle.sector_of(0.001, -1.0, 256) == 192 and oracle.sector_of(1.0, -0.001, 256) == 255
+    rec = np.full(abi.TERRAIN_HORIZON_FLOATS, np.nan, np.float32)
+    rec[0] = 0.04
+    for sector in range(abi.TERRAIN_SECTORS):
+        rec[1 + sector * 13] = -np.inf
+    rec[1] = 0.04
+    rec[1 + 1 + 2 * 1], rec[1 + 2 + 2 * 1] = 1000.0, 0.04
+    behind = oracle.horizon_loss(rec, 32, 5000.0, 10.0, 150.0)
+    expected = oracle.edge_loss(1000.0, float(np.float32(0.04)) * 1000.0, math.hypot(5000.0, 10.0), 150.0)
+    assert behind > 0.0 and abs(behind - expected) < 1e-9
+    assert oracle.horizon_loss(rec, 32, -5000.0, -10.0, 150.0) == 0.0
+    assert oracle.horizon_loss(rec, 32, 800.0, 1.0, 10.0) == 0.0
+    assert oracle.horizon_loss(rec, 32, 5000.0, 10.0, 400.0) == 0.0
+
+
+def test_screening_lowers_only_the_sel():
+    """With horizons, the SEL energy of each receiver drops or stays (the larger of the losses less
+    the lateral attenuation already taken), LAmax, N65/N70 and the histogram do not change, and a
+    receiver with no horizon reads the unscreened values."""
+    sc = scenes.aircraft_scene(scale=0.02)
+    scr = scenes.aircraft_horizons(sc)
+    pts = np.arange(0, sc.points_xy.shape[0], 7)
+    sub = dict(scr, horizon_of=scr["horizon_of"][pts])
+    free = oracle.aircraft_points(sc, sc.points_xy[pts])
+    screened = oracle.aircraft_points(sc, sc.points_xy[pts], screening=sub)
+    assert np.all(screened["period_energy"] <= free["period_energy"] * (1 + 1e-12))
+    assert np.any(screened["period_energy"] < free["period_energy"] * 0.999)
+    for k in ("n65", "n70", "hist"):
+        assert np.array_equal(screened[k], free[k])
+    none = sub["horizon_of"] == abi.NO_HORIZON
+    assert none.any()
+    assert np.array_equal(screened["period_energy"][none], free["period_energy"][none])

@@ -107,3 +107,105 @@ def test_endpoints_excluded_and_symmetry():
     _, a = oracle.crossings(O, (50.0, 107.0), (210.0, 113.0))
     _, b = oracle.crossings(O, (210.0, 113.0), (50.0, 107.0))
     np.testing.assert_allclose(np.sort(a["t"]), np.sort(1.0 - b["t"]), atol=1e-12)
+
+
+def _coarse_height(T, x, y):
+    """Bilinear height of the coarse lattice at (x, y), numpy."""
+    cp, rp = (x - T.origin_x_m) / T.spacing_m, (y - T.origin_y_m) / T.spacing_m
+    c, r = int(math.floor(cp)), int(math.floor(rp))
+    fc, fr = cp - c, rp - r
+    h = -500.0 + (T.nodes & 0xFFFF).astype(np.float64) * 0.2
+    c1, r1 = c + (fc > 0), r + (fr > 0)
+    return ((1 - fr) * ((1 - fc) * h[r, c] + fc * h[r, c1]) + fr * ((1 - fc) * h[r1, c] + fc * h[r1, c1]))
+
+
+def _detail(T, x, y):
+    """The fine detail at (x, y) where its block has fine terrain, numpy (bilinear on the fine
+    lattice, which is the line's linear reading on a fine line)."""
+    cp, rp = (x - T.origin_x_m) / T.spacing_m, (y - T.origin_y_m) / T.spacing_m
+    by = min(max(int(math.floor(rp / abi.FINE_BLOCK_CELLS)), 0), T.fine_blocks.shape[0] - 1)
+    bx = min(max(int(math.floor(cp / abi.FINE_BLOCK_CELLS)), 0), T.fine_blocks.shape[1] - 1)
+    if not T.fine_blocks[by, bx]:
+        return 0.0
+    fr, fc = max(rp * abi.FINE_PER_NODE, 0.0), max(cp * abi.FINE_PER_NODE, 0.0)
+    R, Cn = T.fine_detail.shape
+    r, c = min(int(math.floor(fr)), R - 2), min(int(math.floor(fc)), Cn - 2)
+    a, b = fr - r, fc - c
+    d = T.fine_detail.astype(np.float64)
+    return ((1 - a) * ((1 - b) * d[r, c] + b * d[r, c + 1]) + a * ((1 - b) * d[r + 1, c] + b * d[r + 1, c + 1])) * 0.2
+
+
+def _lines(lo_pos, hi_pos, t_from, t_to):
+    """t of every integer line strictly between fractions t_from and t_to of a ray from lo_pos
+    (t = 0) to hi_pos (t = 1), in lattice units."""
+    out = []
+    if hi_pos == lo_pos:
+        return out
+    for k in range(int(math.floor(min(lo_pos, hi_pos))) - 1, int(math.ceil(max(lo_pos, hi_pos))) + 2):
+        t = (k - lo_pos) / (hi_pos - lo_pos)
+        if t_from < t < t_to:
+            out.append(t)
+    return out
+
+
+def _fine_near(T, a, b):
+    for x in (min(a[0], b[0]), max(a[0], b[0])):
+        for y in (min(a[1], b[1]), max(a[1], b[1])):
+            cp, rp = (x - T.origin_x_m) / T.spacing_m, (y - T.origin_y_m) / T.spacing_m
+            by = min(max(int(math.floor(rp / abi.FINE_BLOCK_CELLS)), 0), T.fine_blocks.shape[0] - 1)
+            bx = min(max(int(math.floor(cp / abi.FINE_BLOCK_CELLS)), 0), T.fine_blocks.shape[1] - 1)
+            if T.fine_blocks[by, bx]:
+                return True
+    return False
+
+
+def test_fine_terrain_profile_is_every_line_of_each_stretch(tile):
+    """With fine terrain (C3'): within 100 m of either end, where a block under that stretch has
+    fine terrain, every line of the 1/6 lattice; every coarse line in between; a vertex where the
+    lattice changes; each vertex's height the coarse bilinear ground, plus on the fine lattice and at
+    the ends and switches (Ground::at) the fine detail of its block (numpy bilinear, brute force)."""
+    T = scenes.with_fine_terrain(tile.scene.terrain)
+    rng = np.random.default_rng(8)
+    ext = tile.grid_shape[0] * 10.0
+    fine_rays = 0
+    for trial in range(60):
+        s = rng.uniform(50, ext - 50, 2)
+        length = 60.0 if trial % 5 == 0 else rng.uniform(150, 900)
+        ang = rng.uniform(0, 2 * math.pi)
+        r = s + length * np.array([math.cos(ang), math.sin(ang)])
+        if not (0 < r[0] < ext and 0 < r[1] < ext):
+            continue
+        failed, p = oracle.fill_profile(T, s, r)
+        assert failed == 0
+        L = math.hypot(*(r - s))
+        along = lambda t: s + t * (r - s)
+        if L <= 200.0:
+            stretches = [(0.0, 1.0, 6 if _fine_near(T, s, r) else 1)]
+        else:
+            e = 100.0 / L
+            stretches = []
+            for a, b, sc in ((0.0, e, 6 if _fine_near(T, s, along(e)) else 1), (e, 1 - e, 1),
+                             (1 - e, 1.0, 6 if _fine_near(T, along(1 - e), r) else 1)):
+                if stretches and stretches[-1][2] == sc:
+                    stretches[-1] = (stretches[-1][0], b, sc)
+                else:
+                    stretches.append((a, b, sc))
+        expected, fine_at = [], []
+        for a, b, sc in stretches:
+            unit = T.spacing_m / sc
+            ts = _lines((s[0] - T.origin_x_m) / unit, (r[0] - T.origin_x_m) / unit, a, b)
+            ts += _lines((s[1] - T.origin_y_m) / unit, (r[1] - T.origin_y_m) / unit, a, b)
+            ts = sorted(set(np.round(ts, 12)))
+            expected += ts
+            fine_at += [sc == 6] * len(ts)
+            if b < 1.0:
+                expected.append(b)
+                fine_at.append(True)  # Ground::at reads the fine height
+            fine_rays += sc == 6
+        assert len(p["t"]) == len(expected) + 2, trial
+        np.testing.assert_allclose(p["t"][1:-1], expected, atol=1e-9)
+        for t, z, fine in zip(p["t"], p["ground_m"], [True] + fine_at + [True]):
+            x, y = along(t)
+            want = _coarse_height(T, x, y) + (_detail(T, x, y) if fine else 0.0)
+            assert abs(want - z) < 1e-6, (trial, t)
+    assert fine_rays >= 20
