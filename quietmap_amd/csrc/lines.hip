@@ -65,6 +65,7 @@ __global__ __launch_bounds__(256) void line_nodes_kernel(Scene sc, const qm_rece
     divergence[i] = (float)line_divergence_factor(g);
 }
 
+template <bool FINE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void line_rays_kernel(
     Scene sc, BandTable B, const qm_receiver* receivers, const qm_line_piece* pieces, const uint32_t* pair_point,
     const uint32_t* pair_piece, const uint32_t* ray_pair, const uint32_t* ray_first, uint32_t n_rays,
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     opt.obstacles = node.obstacles_on_ray != 0;
     opt.weight = node.weight_rad * divergence[i];
     PathResult r;
-    evaluate_path(sc, B, receivers[pair_point[i]], src, w, r, opt);
+    evaluate_path<FINE>(sc, B, receivers[pair_point[i]], src, w, r, opt);
     if (r.failed) {
         atomicOr(&failed[i], r.failed);
         return;
@@ -237,9 +238,14 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     if (total) L_TRY(hipMemcpy(d_ray_pair, ray_pair.data(), 4 * total, hipMemcpyHostToDevice));
     L_TRY(hipMemcpy(d_ray_first, first.data(), 4 * n_pairs, hipMemcpyHostToDevice));
     if (total) {
-        hipLaunchKernelGGL(line_rays_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0, square->scene,
-                           gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first, (uint32_t)total,
-                           d_nodes, d_div, d_sums, d_failed, d_pair_bands);
+        if (square->scene.terrain.fine_detail)
+            hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0,
+                               square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
+                               (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
+        else
+            hipLaunchKernelGGL(line_rays_kernel<false>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0,
+                               square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
+                               (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
         L_TRY(hipGetLastError());
     }
     if (d_bands) {

@@ -58,6 +58,9 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 // Occupancy target of the path kernels (waves per SIMD): the register budget the compiler gets.
 
 // Gpu::evaluate_pairs' ray transfer (one ray per point pair): sums and point bands.
+// FINE: the square holds fine terrain (its own instantiation, so the coarse kernel keeps its
+// registers).
+template <bool FINE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WA
     PathResult r;
     const uint32_t p = pair_point[i];
     const qm_source src = sources[pair_source[i]];
-    evaluate_path(sc, B, receivers[p], src, w, r);
+    evaluate_path<FINE>(sc, B, receivers[p], src, w, r);
     failed[i] = r.failed;
     float* out = sums + (size_t)i * QM_PAIR_SUMS;
     for (int q = 0; q < PERIODS; ++q) {
@@ -102,6 +105,7 @@ __device__ uint32_t next_tile(const GridArgs& g, int xcd) {
     return g.n_tiles;
 }
 
+template <bool FINE>
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
     __shared__ uint32_t s_tile;
     // The two states' scalars of every work-item in LDS: touched at every candidate, they stay
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
                 const double dx = sx - rx, dy = sy - ry;
                 if (!valid || dx * dx + dy * dy > cut2) continue;
                 PathResult r;
-                evaluate_path(sc, B, rcv, sg.sources[s], w, r);
+                evaluate_path<FINE>(sc, B, rcv, sg.sources[s], w, r);
                 ++paths;
                 if (r.failed) {
                     ++fails;
@@ -336,6 +340,21 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     sq->scene.terrain = *terrain;
     sq->scene.terrain.nodes = d_nodes;
     sq->scene.terrain.canopy_m = d_canopy;
+    sq->scene.terrain.fine_detail = nullptr;
+    sq->scene.terrain.fine_blocks = nullptr;
+    if (terrain->fine_detail) {
+        if (!terrain->fine_blocks) return cleanup(fail(err, errlen, "fine terrain without its blocks"));
+        const size_t fine = (size_t)(QM_FINE_PER_NODE * (terrain->rows - 1) + 1) *
+                            (size_t)(QM_FINE_PER_NODE * (terrain->columns - 1) + 1);
+        const size_t blocks = (size_t)((terrain->rows - 2) / QM_FINE_BLOCK_CELLS + 1) *
+                              (size_t)((terrain->columns - 2) / QM_FINE_BLOCK_CELLS + 1);
+        int16_t* d_detail;
+        uint8_t* d_blocks;
+        if (upload(sq, terrain->fine_detail, fine, &d_detail, err, errlen)) return cleanup(1);
+        if (upload(sq, terrain->fine_blocks, blocks, &d_blocks, err, errlen)) return cleanup(1);
+        sq->scene.terrain.fine_detail = d_detail;
+        sq->scene.terrain.fine_blocks = d_blocks;
+    }
     // Obstacles: validate the index before it reaches the device.
     const size_t cells = (size_t)obstacles->cells_x * obstacles->cells_y;
     if (obstacles->cell_first_run[cells] != obstacles->runs_count)
@@ -483,10 +502,16 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                           errlen))
         return 1;
     const unsigned blocks = (unsigned)((n_pairs + 255) / 256);
-    hipLaunchKernelGGL(pairs_kernel, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
-                       (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
-                       (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                       (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr));
+    if (square->scene.terrain.fine_detail)
+        hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
+                           (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
+                           (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
+                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr));
+    else
+        hipLaunchKernelGGL(pairs_kernel<false>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
+                           (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
+                           (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
+                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr));
     QM_TRY(hipGetLastError());
     QM_TRY(hipMemcpyAsync(sums, d_sums.p, sizeof(float) * n_pairs * QM_PAIR_SUMS, hipMemcpyDeviceToHost, stream));
     QM_TRY(hipMemcpyAsync(failed, d_failed.p, sizeof(uint32_t) * n_pairs, hipMemcpyDeviceToHost, stream));
@@ -553,11 +578,19 @@ int qm_paint_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_rec
     int cus = 0;
     QM_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal));
     int per_cu = 0;
-    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(paint_kernel), GROUP, 0));
+    const bool fine = square->scene.terrain.fine_detail != nullptr;
+    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, fine ? reinterpret_cast<const void*>(paint_kernel<true>) : reinterpret_cast<const void*>(paint_kernel<false>),
+        GROUP, 0));
     unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
     groups = std::min<unsigned>(groups, g.n_tiles);
     groups = std::max<unsigned>(groups, 1);
-    hipLaunchKernelGGL(paint_kernel, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands, square->grid, g);
+    if (fine)
+        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+                           square->grid, g);
+    else
+        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+                           square->grid, g);
     QM_TRY(hipGetLastError());
     QM_TRY(hipFreeAsync(d_counters, stream));
     return 0;

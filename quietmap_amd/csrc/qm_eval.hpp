@@ -186,18 +186,104 @@ QM_DEV void foliage_sample(Foliage& f, const PathScratch& w, float x, float grou
     f.prev_canopy = canopy;
 }
 
+// The ray's stretches by lattice (popup/src/scene.rs:242-309 Ground::fill_profile): within
+// QM_FINE_ENDS_M of either end the fine lattice where a block under that stretch has fine terrain,
+// the coarse one elsewhere, neighbours on one lattice joined. `to` ends each stretch (the last 1).
+struct StretchPlan {
+    int count;
+    int scale[3];
+    float to[3];
+    double to_d[3];
+};
+
+// popup/src/scene.rs Ground::fine_near: a block at a corner of the stretch's box has fine terrain.
+QM_DEV bool fine_near(const qm_terrain& T, double ax, double ay, double bx, double by) {
+    const double inv = 1.0 / T.spacing_m;
+    const double xs[2] = {fmin(ax, bx), fmax(ax, bx)}, ys[2] = {fmin(ay, by), fmax(ay, by)};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            if (fine_block(T, (xs[i] - T.origin_x_m) * inv, (ys[j] - T.origin_y_m) * inv)) return true;
+    return false;
+}
+
+QM_DEV StretchPlan stretch_plan(const qm_terrain& T, double sx, double sy, double rx, double ry, double length) {
+    StretchPlan P;
+    P.count = 1;
+    P.scale[0] = 1;
+    P.to[0] = 1.0f;
+    P.to_d[0] = 1.0;
+    if (!T.fine_detail) return P;
+    if (length <= 2.0 * QM_FINE_ENDS_M) {
+        P.scale[0] = fine_near(T, sx, sy, rx, ry) ? QM_FINE_PER_NODE : 1;
+        return P;
+    }
+    const double e = QM_FINE_ENDS_M / length;
+    const double ax = sx + e * (rx - sx), ay = sy + e * (ry - sy);
+    const double bx = sx + (1.0 - e) * (rx - sx), by = sy + (1.0 - e) * (ry - sy);
+    const int scales[3] = {fine_near(T, sx, sy, ax, ay) ? QM_FINE_PER_NODE : 1, 1,
+                           fine_near(T, bx, by, rx, ry) ? QM_FINE_PER_NODE : 1};
+    const double ends[3] = {e, 1.0 - e, 1.0};
+    P.count = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (P.count > 0 && P.scale[P.count - 1] == scales[k]) {
+            P.to_d[P.count - 1] = ends[k];
+        } else {
+            P.scale[P.count] = scales[k];
+            P.to_d[P.count] = ends[k];
+            ++P.count;
+        }
+    }
+    for (int k = 0; k < P.count; ++k) P.to[k] = (float)P.to_d[k];
+    return P;
+}
+
+// The switch vertex at fraction `to` of the ray (Ground::at where the lattice changes).
+QM_DEV uint32_t switch_vertex(const qm_terrain& T, double sx, double sy, double rx, double ry, double to,
+                              GroundSample& out) {
+    const double inv = 1.0 / T.spacing_m;
+    const double x = sx - to * (sx - rx), y = sy - to * (sy - ry);
+    return ground_at_fine(T, (x - T.origin_x_m) * inv, (y - T.origin_y_m) * inv, out);
+}
+
+// The ray's ends and stretches, for terrain_bracket on a fine scene.
+struct RayEnds {
+    double sx, sy, rx, ry;
+    StretchPlan plan;
+};
+
 // terrain_at and ground_factor_at of physics/src/ray.rs:265-283 at x = t * length, from the two
-// lattice-line vertices bracketing it (platform-clamped as the terrain polyline is).
+// walk vertices bracketing it (platform-clamped as the terrain polyline is). With FINE, within the
+// stretch that holds t, a stretch's switch vertex standing in where no line lies between.
+template <bool FINE = false>
 QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, float length, float platform,
                                 float source_ground, const GroundSample& start, const GroundSample& end, float& z_out,
-                                float& g_out) {
+                                float& g_out, const RayEnds* ray = nullptr) {
     float tp, tn;
     GroundSample a, b;
     bool ea, eb;
-    uint32_t failed = bracket_vertex(T, L, t, length, false, tp, a, ea);
-    failed |= bracket_vertex(T, L, t, length, true, tn, b, eb);
-    if (ea) a = start;
-    if (eb) b = end;
+    uint32_t failed;
+    if (FINE && ray && !(ray->plan.count == 1 && ray->plan.scale[0] == 1)) {
+        const StretchPlan& P = ray->plan;
+        int k = 0;
+        while (k + 1 < P.count && !(t < P.to[k])) ++k;
+        const float from = k ? P.to[k - 1] : 0.0f, to = P.to[k];
+        const Lattice Lk = lattice_of(T, ray->sx, ray->sy, ray->rx, ray->ry, P.scale[k], from);
+        failed = bracket_vertex<true>(T, Lk, t, length, false, tp, a, ea, from, to);
+        failed |= bracket_vertex<true>(T, Lk, t, length, true, tn, b, eb, from, to);
+        if (ea) {
+            if (k == 0) a = start;
+            else failed |= switch_vertex(T, ray->sx, ray->sy, ray->rx, ray->ry, P.to_d[k - 1], a);
+        }
+        if (eb) {
+            if (k + 1 == P.count) b = end;
+            else failed |= switch_vertex(T, ray->sx, ray->sy, ray->rx, ray->ry, P.to_d[k], b);
+        }
+    } else {
+        failed = bracket_vertex(T, L, t, length, false, tp, a, ea);
+        failed |= bracket_vertex(T, L, t, length, true, tn, b, eb);
+        if (ea) a = start;
+        if (eb) b = end;
+    }
     if (failed) return failed;
     const float x = t * length;
     const float xa = tp * length, xb = tn * length;
@@ -218,7 +304,7 @@ QM_DEV qm_receiver receiver_of(const Scene& sc, const qm_point& p) {
     const qm_terrain& T = sc.terrain;
     GroundSample g;
     const double inv = 1.0 / T.spacing_m;
-    r.failed = ground_at(T, (p.x_m - T.origin_x_m) * inv, (p.y_m - T.origin_y_m) * inv, g);
+    r.failed = ground_at_fine(T, (p.x_m - T.origin_x_m) * inv, (p.y_m - T.origin_y_m) * inv, g);
     r.altitude_m = r.failed ? 0.0f : g.z + (float)QM_RECEIVER_HEIGHT_M;
     const qm_weather& W = sc.weather;
     const double yy = clampd((p.y_m - W.origin_y_m) / W.spacing_m, 0.0, (double)(W.rows - 1));
@@ -256,6 +342,9 @@ struct RayOptions {
     float weight = -1.0f;
 };
 
+// FINE: the scene may hold fine terrain (qm_terrain.fine_detail): the walk's stretches near the
+// ends go over the fine lattice (popup/src/scene.rs:242-309); without it the coarse walk alone.
+template <bool FINE = false>
 QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver& rcv, const qm_source& src,
                           PathScratch& w, PathResult& out, const RayOptions& opt = RayOptions{}) {
     uint32_t failed = rcv.failed;
@@ -286,8 +375,15 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const Lattice L = lattice_of(T, sx, sy, rx, ry);
     const double inv = 1.0 / T.spacing_m;
     GroundSample s0, r1;
-    failed |= ground_at(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
-    failed |= ground_at(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+    RayEnds ray;
+    if (FINE) {
+        ray = RayEnds{sx, sy, rx, ry, stretch_plan(T, sx, sy, rx, ry, length_d)};
+        failed |= ground_at_fine(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
+        failed |= ground_at_fine(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+    } else {
+        failed |= ground_at(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
+        failed |= ground_at(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+    }
     if (failed & QM_FAILED_NO_TERRAIN) {
         out.failed = failed;
         return;
@@ -306,7 +402,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         const float t = (float)c.t;
         c.x = (float)(c.t * (double)H);
         float z, g;
-        failed |= terrain_bracket(T, L, t, H, platform, source_ground, s0, r1, z, g);
+        failed |= terrain_bracket<FINE>(T, L, t, H, platform, source_ground, s0, r1, z, g, FINE ? &ray : nullptr);
         c.top = z + c.height_m;
         c.z_at = z;  // terrain under the wall, until the walk resolves the profile there
         c.g_at = g;
@@ -417,10 +513,19 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
     crest_vertex(w, P, cg, tp, n, S, R, H, 0.0f, s0.z, failed);
 
-    int next_column = L.first_column, next_row = L.first_row, south = L.start_south;
+    // The walk's lattice: the coarse one, or with FINE each stretch's in turn.
+    Lattice WL = L;
+    int stretch = 0;
+    float stretch_to = 1.0f;
+    if (FINE && !(ray.plan.count == 1 && ray.plan.scale[0] == 1)) {
+        WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[0], 0.0f);
+        stretch_to = ray.plan.to[0];
+    }
+    int next_column = WL.first_column, next_row = WL.first_row, south = WL.start_south;
     const float inf = __builtin_inff();
-    // Every line crossed is one vertex: the walk ends within |columns| + |rise| + 2 steps.
-    int budget = (int)(fabsf(L.columns) + fabsf(L.rise)) + 4;
+    // Every line crossed is one vertex: a stretch ends within |columns| + |rise| + 2 steps of its
+    // lattice over its share of the ray.
+    int budget = (int)((fabsf(WL.columns) + fabsf(WL.rise)) * stretch_to) + 4;
     bool last = false;
     while (!last) {
         if (--budget < 0) {
@@ -429,18 +534,33 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         }
         float t;
         GroundSample v;
-        const float t_column = L.columns != 0.0f ? t_of_column(L, next_column) : inf;
-        const float t_row = L.rise != 0.0f ? t_of_row(L, next_row) : inf;
+        const float t_column = WL.columns != 0.0f ? t_of_column(WL, next_column) : inf;
+        const float t_row = WL.rise != 0.0f ? t_of_row(WL, next_row) : inf;
         t = fminf(t_column, t_row);
-        if (t >= 1.0f) {
-            t = 1.0f;
-            v = r1;
-            last = true;
+        if (t >= stretch_to) {
+            if (!FINE || stretch_to >= 1.0f) {
+                t = 1.0f;
+                v = r1;
+                last = true;
+            } else {
+                // The switch vertex where the lattice changes, then the next stretch's lattice.
+                t = stretch_to;
+                failed |= switch_vertex(T, sx, sy, rx, ry, ray.plan.to_d[stretch], v);
+                const float from = stretch_to;
+                ++stretch;
+                stretch_to = ray.plan.to[stretch];
+                WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[stretch], from);
+                next_column = WL.first_column;
+                next_row = WL.first_row;
+                south = WL.start_south;
+                budget += (int)((fabsf(WL.columns) + fabsf(WL.rise)) * (stretch_to - from)) + 4;
+                if (failed & QM_FAILED_NO_TERRAIN) break;
+            }
         } else {
-            if (t_row <= t_column) failed |= row_vertex(T, L, next_row, t, v);
-            else failed |= column_vertex(T, L, next_column, south, t, v);
+            if (t_row <= t_column) failed |= row_vertex<FINE>(T, WL, next_row, t, v);
+            else failed |= column_vertex<FINE>(T, WL, next_column, south, t, v);
             if (t_row <= t_column) {
-                if (L.rise > 0.0f) {
+                if (WL.rise > 0.0f) {
                     south = next_row;
                     next_row += 1;
                 } else {
@@ -448,7 +568,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
                     next_row -= 1;
                 }
             }
-            if (t_column <= t_row) next_column += L.column_step;
+            if (t_column <= t_row) next_column += WL.column_step;
             if (failed & QM_FAILED_NO_TERRAIN) break;
         }
         const float x = t * H;

@@ -115,14 +115,19 @@ QM_DEV uint32_t ground_at(const qm_terrain& T, double cp, double rp, GroundSampl
 
 // The ray's lattice geometry: position of the source (ca, ra) in lattice units, the offsets to
 // the receiver, and the first lines strictly ahead of the source.
+// `scale`: the lattice scale times as fine as the coarse one (1, or QM_FINE_PER_NODE on a fine
+// stretch); the first lines are those strictly after fraction `from` of the ray.
 struct Lattice {
     float ca, ra, columns, rise, inv_columns, inv_rise;
     int first_column, column_step, first_row, row_step, start_south;
+    int scale;
 };
 
-QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, double ry) {
+QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, double ry, int scale = 1,
+                          float from = 0.0f) {
     Lattice L;
-    const double inv = 1.0 / T.spacing_m;
+    L.scale = scale;
+    const double inv = (double)scale / T.spacing_m;
     const double ca = (sx - T.origin_x_m) * inv, cb = (rx - T.origin_x_m) * inv;
     const double ra = (sy - T.origin_y_m) * inv, rb = (ry - T.origin_y_m) * inv;
     L.ca = (float)ca;
@@ -131,41 +136,111 @@ QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, 
     L.rise = (float)(rb - ra);
     L.inv_columns = L.columns != 0.0f ? 1.0f / L.columns : 0.0f;
     L.inv_rise = L.rise != 0.0f ? 1.0f / L.rise : 0.0f;
+    const float c0 = L.ca + from * L.columns, r0 = L.ra + from * L.rise;
     if (L.columns > 0.0f) {
-        L.first_column = (int)floorf(L.ca) + 1;
+        L.first_column = (int)floorf(c0) + 1;
         L.column_step = 1;
     } else {
-        L.first_column = (int)ceilf(L.ca) - 1;
+        L.first_column = (int)ceilf(c0) - 1;
         L.column_step = -1;
     }
     if (L.rise > 0.0f) {
-        L.first_row = (int)floorf(L.ra) + 1;
+        L.first_row = (int)floorf(r0) + 1;
         L.row_step = 1;
-        L.start_south = (int)floorf(L.ra);
+        L.start_south = (int)floorf(r0);
     } else if (L.rise < 0.0f) {
-        L.first_row = (int)ceilf(L.ra) - 1;
+        L.first_row = (int)ceilf(r0) - 1;
         L.row_step = -1;
-        L.start_south = (int)ceilf(L.ra) - 1;
+        L.start_south = (int)ceilf(r0) - 1;
     } else {
         L.first_row = 0;
         L.row_step = 0;
-        L.start_south = (int)floorf(L.ra);
+        L.start_south = (int)floorf(r0);
     }
     return L;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fine terrain (tiles/src/terrain_fine.rs; paint-gpu/kernels/terrain.cuh:91-103, 155-175 and
+// terrain_walk.cuh:59-160): detail in QM_DETAIL_STEP_M steps on the lattice QM_FINE_PER_NODE times
+// as fine, read in the blocks of QM_FINE_BLOCK_CELLS square cells flagged as having it.
+QM_DEV bool fine_block(const qm_terrain& T, double cp, double rp) {
+    if (!T.fine_detail) return false;
+    const int bx_n = ((int)T.columns - 2) / QM_FINE_BLOCK_CELLS + 1;
+    const int by_n = ((int)T.rows - 2) / QM_FINE_BLOCK_CELLS + 1;
+    const int bx = min(max((int)floor(cp / QM_FINE_BLOCK_CELLS), 0), bx_n - 1);
+    const int by = min(max((int)floor(rp / QM_FINE_BLOCK_CELLS), 0), by_n - 1);
+    return T.fine_blocks[by * bx_n + bx] != 0;
+}
+
+QM_DEV uint32_t fine_rows(const qm_terrain& T) { return QM_FINE_PER_NODE * (T.rows - 1) + 1; }
+QM_DEV uint32_t fine_columns(const qm_terrain& T) { return QM_FINE_PER_NODE * (T.columns - 1) + 1; }
+
+// The detail's bilinear value (m) at fine position (row, column).
+QM_DEV float detail_at(const qm_terrain& T, float row, float column) {
+    const uint32_t rows = fine_rows(T), columns = fine_columns(T);
+    row = fmaxf(row, 0.0f);
+    column = fmaxf(column, 0.0f);
+    const uint32_t r = min((uint32_t)floorf(row), rows - 2), c = min((uint32_t)floorf(column), columns - 2);
+    const float fr = row - (float)r, fc = column - (float)c;
+    const int16_t* d = T.fine_detail;
+    const size_t at = (size_t)r * columns + c;
+    const float sum = (1.0f - fr) * ((1.0f - fc) * (float)d[at] + fc * (float)d[at + 1]) +
+                      fr * ((1.0f - fc) * (float)d[at + columns] + fc * (float)d[at + columns + 1]);
+    return sum * (float)QM_DETAIL_STEP_M;
+}
+
+// popup/src/scene.rs Ground::at: the coarse ground plus, in a block with fine terrain, its detail.
+QM_DEV uint32_t ground_at_fine(const qm_terrain& T, double cp, double rp, GroundSample& out) {
+    const uint32_t failed = ground_at(T, cp, rp, out);
+    if (!failed && fine_block(T, cp, rp))
+        out.z += detail_at(T, (float)(QM_FINE_PER_NODE * rp), (float)(QM_FINE_PER_NODE * cp));
+    return failed;
+}
+
+// A vertex of the fine walk (popup/src/scene.rs Ground::fine_sample): on fine row line `line` (or
+// fine column line `line` between fine rows `south` and south + 1), `position` along the fine
+// lattice and `f` the fraction between the line's two fine nodes: the coarse cell's sample at the
+// vertex's place plus, in a block with fine terrain, the detail between those nodes.
+QM_DEV uint32_t fine_vertex(const qm_terrain& T, bool on_row, int line, int south, float position, float f,
+                            GroundSample& out) {
+    const int rows = (int)fine_rows(T), columns = (int)fine_columns(T);
+    const int r0 = on_row ? line : south;
+    const int c0 = on_row ? (int)floorf(position) : line;
+    const int r1 = on_row ? r0 : r0 + 1, c1 = on_row ? c0 + 1 : c0;
+    if (r0 < 0 || c0 < 0 || r0 >= rows || c0 >= columns || (f > 0.0f && (r1 >= rows || c1 >= columns)))
+        return QM_FAILED_NO_TERRAIN;
+    const double rp = (on_row ? (double)r0 : (double)r0 + (double)f) / QM_FINE_PER_NODE;
+    const double cp = (on_row ? (double)c0 + (double)f : (double)c0) / QM_FINE_PER_NODE;
+    const uint32_t failed = ground_at(T, cp, rp, out);
+    if (failed) return failed;
+    if (fine_block(T, cp, rp)) {
+        const int16_t* d = T.fine_detail;
+        float d0 = f < 1.0f ? (float)d[(size_t)r0 * columns + c0] : 0.0f;
+        float d1 = f > 0.0f ? (float)d[(size_t)r1 * columns + c1] : 0.0f;
+        if (f <= 0.0f) d1 = d0;
+        if (f >= 1.0f) d0 = d1;
+        out.z += (d0 + f * (d1 - d0)) * (float)QM_DETAIL_STEP_M;
+    }
+    return 0;
 }
 
 QM_DEV float t_of_column(const Lattice& L, int c) { return ((float)c - L.ca) * L.inv_columns; }
 QM_DEV float t_of_row(const Lattice& L, int k) { return ((float)k - L.ra) * L.inv_rise; }
 
-// Sample on row line k at fraction t.
+// Sample on row line k at fraction t (the fine sample on a fine lattice when FINE).
+template <bool FINE = false>
 QM_DEV uint32_t row_vertex(const qm_terrain& T, const Lattice& L, int k, float t, GroundSample& out) {
     const float c = L.ca + t * L.columns;
     const float west = floorf(c);
+    if (FINE && L.scale != 1) return fine_vertex(T, true, k, 0, c, c - west, out);
     return edge_sample(T, k, (int)west, k, (int)west + 1, c - west, out);
 }
 // Sample on column line c at fraction t, the cell's south row given.
+template <bool FINE = false>
 QM_DEV uint32_t column_vertex(const qm_terrain& T, const Lattice& L, int c, int south, float t, GroundSample& out) {
     const float r = L.ra + t * L.rise;
+    if (FINE && L.scale != 1) return fine_vertex(T, false, c, south, r, clampf(r - (float)south, 0.0f, 1.0f), out);
     return edge_sample(T, south, c, south + 1, c, clampf(r - (float)south, 0.0f, 1.0f), out);
 }
 
@@ -173,10 +248,14 @@ QM_DEV uint32_t column_vertex(const qm_terrain& T, const Lattice& L, int c, int 
 // lattice walk visits every line strictly between the ends; this finds the last line at or before
 // t (`after` false) or the first strictly after t (`after` true) of each family and reads it.
 // Returns the vertex fraction (0 or 1 for the ends, which the caller samples).
+// With FINE the lattice is the stretch's that holds t, [from, to) of the ray, and a vertex at a
+// stretch's bound (`from` before, `to` after) that is not an end is returned as is_end with t_out
+// at that bound, for the caller to read (Ground::at, the walk's switch vertex).
+template <bool FINE = false>
 QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, float length, bool after,
-                               float& t_out, GroundSample& out, bool& is_end) {
+                               float& t_out, GroundSample& out, bool& is_end, float from = 0.0f, float to = 1.0f) {
     const float x = t * length;
-    float best_t = after ? 1.0f : 0.0f;
+    float best_t = after ? to : from;
     int kind = 0;  // 0 end, 1 column, 2 row
     int line = 0;
     if (L.columns != 0.0f) {
@@ -191,7 +270,7 @@ QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, f
         const bool started = L.column_step > 0 ? c >= L.first_column : c <= L.first_column;
         if (started) {
             const float tc = t_of_column(L, c);
-            if (tc < 1.0f && tc > 0.0f) {
+            if (tc < to && tc > from) {
                 if (after ? tc < best_t : tc > best_t) {
                     best_t = tc;
                     kind = 1;
@@ -211,7 +290,7 @@ QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, f
         const bool started = L.row_step > 0 ? k >= L.first_row : k <= L.first_row;
         if (started) {
             const float tr = t_of_row(L, k);
-            if (tr < 1.0f && tr > 0.0f) {
+            if (tr < to && tr > from) {
                 // At a tie the row line's reading (the walk's rule).
                 if (after ? tr <= best_t : tr >= best_t) {
                     best_t = tr;
@@ -223,11 +302,11 @@ QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, f
     }
     t_out = best_t;
     is_end = kind == 0;
-    if (kind == 2) return row_vertex(T, L, line, best_t, out);
+    if (kind == 2) return row_vertex<FINE>(T, L, line, best_t, out);
     if (kind == 1) {
         const float r = L.ra + best_t * L.rise;
         const int south = L.rise >= 0.0f ? (int)floorf(r) : (int)ceilf(r) - 1;
-        return column_vertex(T, L, line, L.rise == 0.0f ? L.start_south : south, best_t, out);
+        return column_vertex<FINE>(T, L, line, L.rise == 0.0f ? L.start_south : south, best_t, out);
     }
     return 0;
 }

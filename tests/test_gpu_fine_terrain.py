@@ -1,0 +1,111 @@
+"""Fine terrain near the ray ends (the 1/6 lattice within 100 m of either end where a block under
+that stretch has fine terrain: popup/src/scene.rs:242-309, paint-gpu/kernels/terrain_walk.cuh) on
+the card against the FP64 oracle, whose walk is pinned by brute force: pair energies, their
+states and point bands within 0.05 dB, painted Lden within 0.05 dB with the paths counted
+exactly; the fine detail moves some pairs (the test is not vacuous); squares with fine terrain are
+refused where the walk is coarse only (batched tiles)."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from quietmap_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+TOLERANCE_DB = 0.05
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from quietmap_amd.painter import Gpu
+
+    g = Gpu(0)
+    yield g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def fine():
+    sc = scenes.tile_scene(scale=0.06)
+    T = scenes.with_fine_terrain(sc.scene.terrain, share=0.6, amplitude_m=6.0)
+    return sc, abi.Scene(T, sc.scene.obstacles, sc.scene.weather)
+
+
+def db(x):
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(np.asarray(x, np.float64))
+
+
+def test_fine_pairs_match_oracle(gpu, fine):
+    sc, scene = fine
+    print("case: fine terrain pairs", flush=True)
+    square = gpu.upload(scene, sc.sources)
+    coarse = gpu.upload(sc.scene, sc.sources)
+    rng = np.random.default_rng(21)
+    pp = rng.integers(0, sc.points.size, 6000).astype(np.uint32)
+    ps = rng.integers(0, sc.sources.size, 6000).astype(np.uint32)
+    d = np.hypot(sc.sources["x_m"][ps] - sc.points["x_m"][pp], sc.sources["y_m"][ps] - sc.points["y_m"][pp])
+    keep = d <= sc.cutoff_m
+    pp, ps, d = pp[keep], ps[keep], d[keep]
+    assert (d <= 200.0).sum() > 20
+    got = gpu.evaluate_pairs(square, sc.points, pp, ps)
+    want = oracle.evaluate_pairs(scene, sc.points, sc.sources, pp, ps)
+    assert not want["failed"].any()
+    # Only capacity failures (QM_FAILED_CAPACITY | QM_DETAIL_HULL), and rare.
+    bad = got["failed"] != 0
+    assert np.all((got["failed"][bad] & ~np.uint32(4 | 64)) == 0) and bad.mean() < 1e-3
+    keep = ~bad
+    pp, ps = pp[keep], ps[keep]
+    err = np.abs(db(got["sums"][keep]) - db(want["sums"][keep]))
+    err = err[np.isfinite(err)]
+    print(f"fine pairs: n={pp.size} max {err.max():.5f} dB p99.9 {np.quantile(err, 0.999):.5f}", flush=True)
+    assert err.max() <= TOLERANCE_DB
+    if not bad.any():  # a failed pair leaves its point's bands short
+        gb, wb = got["point_bands"].astype(np.float64), want["point_bands"]
+        assert np.array_equal(gb > 0, wb > 0)
+        assert np.abs(db(gb[wb > 0]) - db(wb[wb > 0])).max() <= TOLERANCE_DB
+    plain = gpu.evaluate_pairs(coarse, sc.points, pp, ps)
+    moved = np.abs(db(plain["sums"][keep, :3]) - db(got["sums"][keep, :3]))
+    print(f"fine detail moves pairs by up to {np.nanmax(moved):.2f} dB", flush=True)
+    assert np.nanmax(moved) > 0.1
+
+
+def test_fine_grid_matches_oracle(gpu, fine):
+    sc, scene = fine
+    print("case: fine terrain grid", flush=True)
+    square = gpu.upload(scene, sc.sources)
+    nx, ny = sc.grid_shape
+    got = gpu.paint_grid(square, sc.points, nx, ny, sc.cutoff_m)
+    # The rough synthetic detail (6 m ripples of 40 m) puts more crests near the ends than a
+    # path's hull holds on a few paths: those report QM_FAILED_CAPACITY, as the reference's card
+    # does for its CPU fallback; the rest are compared.
+    share = got["failures"].sum() / got["paths"].sum()
+    print(f"fine grid: {int(got['failures'].sum())} capacity failures in {int(got['paths'].sum())} paths", flush=True)
+    assert share < 1e-4
+    clean = np.flatnonzero(got["failures"] == 0)
+    sample = np.random.default_rng(9).choice(clean, 48, replace=False)
+    want = oracle.paint_points(scene, sc.points[sample], sc.sources, sc.cutoff_m)
+    assert np.array_equal(got["paths"][sample], want["paths"].astype(np.uint32))
+    assert not want["failures"].any()
+    from quietmap_amd.painter import lden_db
+
+    err = np.abs(lden_db(got["period_energy"][sample]) - lden_db(want["period_energy"]))
+    print(f"fine grid Lden: max {err.max():.5f} dB", flush=True)
+    assert err.max() <= TOLERANCE_DB
+
+
+def test_batched_tiles_refuse_fine_terrain(gpu, fine):
+    from quietmap_amd import hip, tiles
+    from quietmap_amd.painter import PainterError
+
+    sc, scene = fine
+    print("case: fine terrain refused by tiles", flush=True)
+    square = gpu.upload(scene, sc.sources)
+    n = 16 * 16
+    d_rec = hip.DeviceBuffer(n * 256)
+    gpu.receivers_device(square, hip.DeviceBuffer.from_array(sc.points[:n]).ptr, n, d_rec.ptr)
+    tile = np.zeros(1, abi.TILE_DTYPE)
+    tile["nx"], tile["ny"], tile["source_count"] = 16, 16, sc.sources.size
+    d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+    with pytest.raises(PainterError):
+        tiles.TilePlan(gpu, square, tile, d_rec.ptr, n, 64, 2, d_e.ptr, d_p.ptr, d_f.ptr)
