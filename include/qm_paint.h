@@ -131,6 +131,16 @@ int qm_pixel_loudness(qm_gpu* gpu, const float* input, size_t input_floats, cons
  * rows of lambda from 1e-4 at 8 a decade by 100 quantiles, FP32. */
 void qm_kurze_table(float* out);
 
+/* Horizons::build and write_f32 of each point (popup/src/aircraft/horizons.rs:138-201; the
+ * reference builds them on the host per card point, paint-gpu/src/aircraft.rs:252-258): the
+ * terrain's and the buildings' QM_HORIZON_FLOATS floats a point, the screening input of
+ * qm_aircraft_grid. At `altitude_m[i]` (null: the painter's receiver altitude, ground + 4 m),
+ * the walls of the point's own footprint no horizon. `failed` per point: the receiver's failure
+ * bits, a crossing search's (capacity) and QM_FAILED_NO_TERRAIN where a wall stands off the
+ * terrain. Host arrays. */
+int qm_horizons(qm_gpu* gpu, const qm_square* square, const qm_point* points, const double* altitude_m,
+                size_t n_points, float* out, uint32_t* failed, char* err, size_t errlen);
+
 /* Gpu::evaluate_pairs for line candidates (paint-gpu/kernels/paint.cu:77-290 pair_rays, the
  * count / scan / list of rays, and ray_transfers; popup/src/evaluate.rs:171-267 source_rays): per
  * pair (point, line piece) the piece's quadrature nodes seen from the point (physics/src/line.rs,

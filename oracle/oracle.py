@@ -105,6 +105,8 @@ def _declare(L):
     L.qmo_sector_of.restype = C.c_int
     L.qmo_horizon_loss.argtypes = [F32, C.c_int, C.c_double, C.c_double, C.c_double]
     L.qmo_horizon_loss.restype = C.c_double
+    L.qmo_horizons.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_uint64, F32]
+    L.qmo_horizons.restype = C.c_int
 
 
 def finite_fraction(along, length, scaled):
@@ -497,3 +499,12 @@ def sector_of(east, north, sectors):
 def horizon_loss(record, sectors, east, north, up):
     r = np.ascontiguousarray(record, np.float32)
     return lib().qmo_horizon_loss(_p(r, F32), int(sectors), float(east), float(north), float(up))
+
+
+def horizons(scene, x, y, altitude_m, own_footprint=0):
+    """Horizons::build at (x, y): QM_HORIZON_FLOATS float32 (terrain's, then buildings') and the
+    failure bits."""
+    out = np.zeros(abi.HORIZON_FLOATS, np.float32)
+    failed = lib().qmo_horizons(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), float(x), float(y),
+                                float(altitude_m), int(own_footprint), _p(out, F32))
+    return out, failed

@@ -39,6 +39,7 @@ EXPORTS = (
     "qm_evaluate_line_pairs",
     "qm_pixel_loudness",
     "qm_kurze_table",
+    "qm_horizons",
     "qm_mlp_new",
     "qm_mlp_params",
     "qm_mlp_free",
@@ -91,6 +92,7 @@ def library():
         lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
         lib.qm_kurze_table.argtypes = [vp]
         lib.qm_kurze_table.restype = None
+        lib.qm_horizons.argtypes = [vp, vp, vp, vp, sz, vp, vp, cp, sz]
         lib.qm_mlp_new.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
         lib.qm_mlp_params.argtypes = [vp, vp, cp, sz]
         lib.qm_mlp_free.argtypes = [vp]
@@ -290,6 +292,22 @@ class Gpu:
                                          _ptr(mix) if mixtures else None, _ptr(nd), err, 512)
         _check(rc, err)
         return (nd, mix) if mixtures else nd
+
+    def horizons(self, square: Square, points, altitude_m=None):
+        """Terrain and building horizons of `points` (Horizons::build, write_f32): floats
+        [n, QM_HORIZON_FLOATS] and failure bits per point; at `altitude_m` per point, or the
+        receiver altitude (ground + 4 m) when None."""
+        pts = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        n = pts.size
+        alt = None if altitude_m is None else np.ascontiguousarray(altitude_m, np.float64)
+        assert alt is None or alt.size == n
+        out = np.zeros((n, abi.HORIZON_FLOATS), np.float32)
+        failed = np.zeros(n, np.uint32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_horizons(self.handle, square.handle, _ptr(pts), _ptr(alt), n, _ptr(out), _ptr(failed),
+                                   err, 512)
+        _check(rc, err)
+        return out, failed
 
     def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False):
         """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands."""
