@@ -26,6 +26,7 @@ extern "C" {
 
 typedef struct qm_gpu qm_gpu;
 typedef struct qm_square qm_square;
+typedef struct qm_flight_boxes qm_flight_boxes;
 
 /* Gpu::new(ordinal) (paint-gpu/src/device.rs:254-420): opens the device and checks that it is a
  * gfx950 with the library's kernels loaded. */
@@ -140,6 +141,28 @@ void qm_kurze_table(float* out);
  * terrain. Host arrays. */
 int qm_horizons(qm_gpu* gpu, const qm_square* square, const qm_point* points, const double* altitude_m,
                 size_t n_points, float* out, uint32_t* failed, char* err, size_t errlen);
+
+/* DeviceFlightBoxes::upload (paint-gpu/src/aircraft.rs:160-205): a square's aircraft boxes as
+ * the host writes them (qm_flight_box), copied to the device; freed by qm_flight_boxes_free. */
+int qm_flight_boxes_upload(qm_gpu* gpu, const qm_flight_box* boxes, size_t n, qm_flight_boxes** out, char* err,
+                           size_t errlen);
+size_t qm_flight_boxes_count(const qm_flight_boxes* boxes);
+void qm_flight_boxes_free(qm_flight_boxes* boxes);
+
+/* Gpu::flight_sums (paint-gpu/src/aircraft.rs:222-350, kernels/aircraft.cuh:260-311): per segment
+ * its boxes within reach, weighted, their SEL energy per period and their passes in the 48 pass
+ * classes (popup/src/aircraft/boxes.rs:265-300 tile_flight_passes), added into sum slot
+ * `segment.sum` of `sums` (n_sums x QM_FLIGHT_SUMS doubles, zeroed first). `horizons` holds the
+ * points' horizon records (Horizons::write_f32 floats, QM_HORIZON_FLOATS from each point's
+ * offset; QM_NO_HORIZONS: unscreened). Records out of range fail the call. Host arrays. */
+int qm_flight_sums(qm_gpu* gpu, const qm_flight_boxes* boxes, const qm_flight_point* points, size_t n_points,
+                   const float* horizons, size_t n_horizon_floats, const qm_flight_segment* segments,
+                   size_t n_segments, double* sums, size_t n_sums, char* err, size_t errlen);
+/* The same with device arrays on `stream` (a hipStream_t; null: the default stream), the sums
+ * not zeroed (they accumulate); the caller keeps every index in range. */
+int qm_flight_sums_device(qm_gpu* gpu, const qm_flight_boxes* boxes, const qm_flight_point* d_points,
+                          const float* d_horizons, const qm_flight_segment* d_segments, size_t n_segments,
+                          double* d_sums, void* stream, char* err, size_t errlen);
 
 /* Gpu::evaluate_pairs for line candidates (paint-gpu/kernels/paint.cu:77-290 pair_rays, the
  * count / scan / list of rays, and ray_transfers; popup/src/evaluate.rs:171-267 source_rays): per

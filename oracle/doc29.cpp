@@ -357,6 +357,263 @@ int hist_bin(double lmax_db) {
     return k >= QM_HIST_BINS - 1 ? QM_HIST_BINS - 1 : (int)k;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Aircraft boxes: physics/src/doc29/box_sums.rs (the sums a box keeps of its pieces and the values
+// they give), boxes.rs (a box's SEL at a receiver through its average pieces) and the card's
+// flight_sums over the host's records (paint-gpu/kernels/aircraft.cuh:260-311, restating
+// popup/src/aircraft/boxes.rs:265-300 tile_flight_passes).
+
+constexpr int GEOMETRY_DISTANCE = 3;  // 1,000 ft (box_sums.rs GEOMETRY_WEIGHT_DISTANCE)
+const double INSTALLATION_SHARE_SLANTS_M[2] = {1000.0 * METRES_PER_FOOT, TAIL_ANCHOR_M};
+
+// segment.rs:173-180 npd_distance_levels: SEL + Delta_V at the ten NPD distances and the tail
+// anchor, and d_lambda at the ten distances.
+struct DistanceLevels {
+    double sel_db[ND], scaled_distance_m[ND], tail_sel_db;
+};
+
+DistanceLevels npd_distance_levels(const Emission& em) {
+    DistanceLevels out;
+    for (int k = 0; k < ND; ++k) {
+        const Reading r = em.read_npd(NPD_DISTANCES_FT[k] * METRES_PER_FOOT);
+        out.sel_db[k] = r.sel_db + em.speed_correction_db;
+        out.scaled_distance_m[k] = r.scaled_distance_m;
+    }
+    out.tail_sel_db = em.read_npd(TAIL_ANCHOR_M).sel_db + em.speed_correction_db;
+    return out;
+}
+
+double level_of(double e) { return e > 0.0 ? 10.0 * std::log10(e) : -std::numeric_limits<double>::infinity(); }
+
+// A box's values (box_sums.rs BoxValues).
+struct BoxValues {
+    double levels_db[3][ND], tail_levels_db[3], scaled_distance_m[ND];
+    double centroid_m[3], axis_rad, gradient, gradient_spread;
+    double piece_length_m[3];
+    double installation_shares[3][2][3];
+};
+
+// box_sums.rs BoxSums: plain sums of the pieces' weighted energies and geometry.
+struct BoxSums {
+    double energy[3][ND] = {}, tail_energy[3] = {}, energy_over_scaled[ND] = {};
+    double weight = 0.0, midpoint[3] = {};
+    double period_weight[3] = {}, period_length[3] = {};
+    double doubled[2] = {}, gradient_direction[2] = {}, squared_gradient[3] = {};
+    double installation_energy[3][2][3] = {};
+
+    void add(const DistanceLevels& lv, uint32_t installation, const double pw[3], const double s[3],
+             const double e[3]) {
+        double unweighted[ND];
+        for (int k = 0; k < ND; ++k) unweighted[k] = std::pow(10.0, lv.sel_db[k] / 10.0);
+        const double tail = std::pow(10.0, lv.tail_sel_db / 10.0);
+        const double total = pw[0] + pw[1] + pw[2];
+        if (total <= 0.0) return;
+        for (int p = 0; p < 3; ++p) {
+            for (int k = 0; k < ND; ++k) energy[p][k] += pw[p] * unweighted[k];
+            tail_energy[p] += pw[p] * tail;
+        }
+        for (int k = 0; k < ND; ++k) energy_over_scaled[k] += total * unweighted[k] / std::max(lv.scaled_distance_m[k], 1.0);
+        const double w = total * unweighted[GEOMETRY_DISTANCE];
+        const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+        const double length = std::hypot(d[0], d[1]);
+        weight += w;
+        for (int a = 0; a < 3; ++a) midpoint[a] += w * 0.5 * (s[a] + e[a]);
+        for (int p = 0; p < 3; ++p) {
+            const double piece = pw[p] * unweighted[GEOMETRY_DISTANCE];
+            period_weight[p] += piece;
+            period_length[p] += piece * length;
+        }
+        if (length > 0.0) {
+            const double c = d[0] / length, sn = d[1] / length;
+            doubled[0] += w * (c * c - sn * sn);
+            doubled[1] += w * 2.0 * sn * c;
+            const double g = d[2] / length;
+            gradient_direction[0] += w * g * c;
+            gradient_direction[1] += w * g * sn;
+            const double sq = w * g * g;
+            squared_gradient[0] += sq * c * c;
+            squared_gradient[1] += sq * sn * sn;
+            squared_gradient[2] += sq * sn * c;
+        }
+        const int slot = installation == QM_INSTALL_WING ? 0 : installation == QM_INSTALL_FUSELAGE ? 1 : 2;
+        for (int p = 0; p < 3; ++p) {
+            installation_energy[p][0][slot] += pw[p] * unweighted[GEOMETRY_DISTANCE];
+            installation_energy[p][1][slot] += pw[p] * tail;
+        }
+    }
+
+    bool values(BoxValues& v) const {
+        if (weight <= 0.0) return false;
+        for (int p = 0; p < 3; ++p) {
+            for (int k = 0; k < ND; ++k) v.levels_db[p][k] = level_of(energy[p][k]);
+            v.tail_levels_db[p] = level_of(tail_energy[p]);
+        }
+        for (int k = 0; k < ND; ++k)
+            v.scaled_distance_m[k] = (energy[0][k] + energy[1][k] + energy[2][k]) / energy_over_scaled[k];
+        double axis = std::fmod(0.5 * std::atan2(doubled[1], doubled[0]), M_PI);
+        if (axis < 0.0) axis += M_PI;
+        v.axis_rad = axis;
+        const double sn = std::sin(axis), c = std::cos(axis);
+        v.gradient = (gradient_direction[0] * c + gradient_direction[1] * sn) / weight;
+        const double mean_square =
+            (squared_gradient[0] * c * c + squared_gradient[1] * sn * sn + 2.0 * squared_gradient[2] * sn * c) / weight;
+        v.gradient_spread = std::sqrt(std::max(mean_square - v.gradient * v.gradient, 0.0));
+        for (int a = 0; a < 3; ++a) v.centroid_m[a] = midpoint[a] / weight;
+        for (int p = 0; p < 3; ++p) v.piece_length_m[p] = period_weight[p] > 0.0 ? period_length[p] / period_weight[p] : 0.0;
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 2; ++q) {
+                const double t = installation_energy[p][q][0] + installation_energy[p][q][1] + installation_energy[p][q][2];
+                for (int k = 0; k < 3; ++k)
+                    v.installation_shares[p][q][k] = t > 0.0 ? installation_energy[p][q][k] / t : (k == 2 ? 1.0 : 0.0);
+            }
+        return true;
+    }
+};
+
+// boxes.rs:99-111 average_length_m: the periods' lengths weighted by their energy at 1,000 ft.
+double average_length_m(const double levels[3][ND], const double piece_length[3]) {
+    double w[3], total = 0.0;
+    for (int p = 0; p < 3; ++p) {
+        w[p] = energy(levels[p][GEOMETRY_DISTANCE]);
+        total += w[p];
+    }
+    if (!(total > 0.0)) return piece_length[0];
+    double sum = 0.0;
+    for (int p = 0; p < 3; ++p) sum += w[p] * piece_length[p];
+    return sum / total;
+}
+
+// boxes.rs:134-147 box_level_db.
+double box_level_db(const double levels[ND], double tail_db, const Position& pos) {
+    const double last = levels[ND - 1];
+    if (pos.slant_m < NPD_LAST_DISTANCE_M) return pos.linear(levels);
+    if (last == -std::numeric_limits<double>::infinity()) return last;
+    auto spherical = [](double d) { return 20.0 * std::log10(d / NPD_LAST_DISTANCE_M); };
+    const double alpha = std::max((last - spherical(TAIL_ANCHOR_M) - tail_db) / (TAIL_ANCHOR_M - NPD_LAST_DISTANCE_M), 0.0);
+    return last - spherical(pos.slant_m) - alpha * (pos.slant_m - NPD_LAST_DISTANCE_M);
+}
+
+// A box at a receiver (boxes.rs AircraftBoxAtReceiver), the axis as its sine and cosine.
+struct BoxAt {
+    double centroid_m[3];
+    double sin_axis, cos_axis, gradient, gradient_spread;
+    double piece_length_m[3];
+    double levels_db[3][ND], tail_levels_db[3], lg_scaled_distance[ND];
+    double installation_shares[3][2][3];
+    bool screened;
+};
+
+// boxes.rs:205-284 average_piece_sel: the SEL energy per period of the average piece of
+// `length_m` at `gradient` and the received installation fractions.
+void average_piece_sel(const BoxAt& b, double gradient, double length_m, const float* horizon, double sel[3],
+                       double fractions[3][3]) {
+    const double half = 0.5 * std::max(length_m, 1.0);
+    const double step[3] = {b.cos_axis * half, b.sin_axis * half, gradient * half};
+    const double s[3] = {b.centroid_m[0] - step[0], b.centroid_m[1] - step[1], b.centroid_m[2] - step[2]};
+    const double e[3] = {b.centroid_m[0] + step[0], b.centroid_m[1] + step[1], b.centroid_m[2] + step[2]};
+    const Closest c = closest_points(s, e);
+    const double slant = norm3(c.on_line);
+    const Position pos = Position::at(slant);
+    const double scaled = std::exp(pos.linear(b.lg_scaled_distance) * std::log(10.0));
+    const double along_m = c.along * c.length_m;
+    const double stretch = std::hypot(gradient, 1.0);
+    const double lateral = lateral_attenuation_db(c.lambda_height_m, c.lateral_m);
+    const double corrections[3] = {installation_energy_gain(QM_INSTALL_WING, c.height_m, slant),
+                                   installation_energy_gain(QM_INSTALL_FUSELAGE, c.height_m, slant),
+                                   installation_energy_gain(QM_INSTALL_PROPELLER, c.height_m, slant)};
+    const double near = INSTALLATION_SHARE_SLANTS_M[0], far = INSTALLATION_SHARE_SLANTS_M[1];
+    const double fraction = std::min(std::max(std::log10(slant / near) / std::log10(far / near), 0.0), 1.0);
+    double terrain = 0.0, buildings = 0.0;
+    if (b.screened && horizon) {
+        terrain = horizon_loss_db(horizon, QM_TERRAIN_SECTORS, c.on_line);
+        buildings = horizon_loss_db(horizon + QM_TERRAIN_HORIZON_FLOATS, QM_BUILDING_SECTORS, c.on_segment);
+    }
+    const double gain = energy((0.0 - std::max(std::max(terrain, buildings) - lateral, 0.0)) - lateral);
+    for (int p = 0; p < 3; ++p) {
+        double shares[3], parts[3], total = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            shares[k] = b.installation_shares[p][0][k] + fraction * (b.installation_shares[p][1][k] - b.installation_shares[p][0][k]);
+            parts[k] = shares[k] * corrections[k];
+            total += parts[k];
+        }
+        double installation = 1.0;
+        if (total > 0.0) {
+            installation = total;
+            for (int k = 0; k < 3; ++k) fractions[p][k] = parts[k] / total;
+        } else {
+            for (int k = 0; k < 3; ++k) fractions[p][k] = shares[k];
+        }
+        bool silent = true;
+        for (int k = 0; k < ND; ++k) silent = silent && b.levels_db[p][k] == -std::numeric_limits<double>::infinity();
+        if (silent) {
+            sel[p] = 0.0;
+            continue;
+        }
+        const double piece = std::max(b.piece_length_m[p] * stretch, 1.0);
+        const double finite = finite_segment_energy_fraction(along_m + 0.5 * (piece - c.length_m), piece, scaled);
+        sel[p] = energy(box_level_db(b.levels_db[p], b.tail_levels_db[p], pos)) * finite * installation * gain;
+    }
+}
+
+// boxes.rs:159-200 box_sel_at_receiver: one average piece, or two at the gradient less and plus
+// its spread, half the energy each, their installation mixes weighed by what each brings.
+void box_sel(const BoxAt& b, double length_m, const float* horizon, double sel[3], double fractions[3][3]) {
+    if (b.gradient_spread <= 0.0) {
+        average_piece_sel(b, b.gradient, length_m, horizon, sel, fractions);
+        return;
+    }
+    double lo[3], hi[3], flo[3][3], fhi[3][3];
+    average_piece_sel(b, b.gradient - b.gradient_spread, length_m, horizon, lo, flo);
+    average_piece_sel(b, b.gradient + b.gradient_spread, length_m, horizon, hi, fhi);
+    for (int p = 0; p < 3; ++p) {
+        sel[p] = 0.5 * (lo[p] + hi[p]);
+        for (int k = 0; k < 3; ++k)
+            fractions[p][k] = lo[p] + hi[p] > 0.0 ? (lo[p] * flo[p][k] + hi[p] * fhi[p][k]) / (lo[p] + hi[p]) : flo[p][k];
+    }
+}
+
+// paint/src/flight_boxes.rs:23-33 FlightWeight::at.
+double flight_weight(uint32_t kind, double distance_m, double coarse_m) {
+    const double fraction = std::min(std::max((distance_m - 1.5 * coarse_m) / (1.5 * coarse_m), 0.0), 1.0);
+    const double far = fraction * fraction * (3.0 - 2.0 * fraction);
+    return kind == QM_FLIGHT_WEIGHT_ONE ? 1.0 : kind == QM_FLIGHT_WEIGHT_FAR ? far : 1.0 - far;
+}
+
+// A host record at a point (aircraft.cuh flight_sums reads it so): the centroid relative to the
+// point, the record's FP32 fields taken as they are.
+BoxAt box_at(const qm_flight_box& r, const qm_flight_point& pt) {
+    BoxAt b;
+    b.centroid_m[0] = r.east_m - pt.x;
+    b.centroid_m[1] = r.north_m - pt.y;
+    b.centroid_m[2] = r.centroid_altitude_m - pt.altitude_m;
+    b.sin_axis = r.sin_axis;
+    b.cos_axis = r.cos_axis;
+    b.gradient = r.gradient;
+    b.gradient_spread = r.gradient_spread;
+    for (int p = 0; p < 3; ++p) {
+        b.piece_length_m[p] = r.piece_length_m[p];
+        b.tail_levels_db[p] = r.tail_levels_db[p];
+        for (int k = 0; k < ND; ++k) b.levels_db[p][k] = r.silent[p] ? -std::numeric_limits<double>::infinity() : (double)r.levels_db[p][k];
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 3; ++k) b.installation_shares[p][q][k] = r.installation_shares[p][q][k];
+    }
+    for (int k = 0; k < ND; ++k) b.lg_scaled_distance[k] = r.lg_scaled_distance[k];
+    b.screened = r.screened != 0;
+    return b;
+}
+
+Class class_of(const qm_aircraft_class& src) {
+    Class c;
+    c.installation = src.installation;
+    c.v_ref_kt = src.v_ref_kt;
+    for (int dep = 0; dep < 2; ++dep) {
+        c.n_rows[dep] = src.rows[dep];
+        for (uint32_t r = 0; r < src.rows[dep]; ++r) c.rows[dep][r].init(src.sel_db[dep][r], src.lamax_db[dep][r], src.v_ref_kt);
+    }
+    return c;
+}
+
 }  // namespace qmo
 
 using namespace qmo;
@@ -502,6 +759,129 @@ int qmo_sector_of(double east, double north, int sectors) { return sector_of(eas
 double qmo_horizon_loss(const float* at, int sectors, double east, double north, double up) {
     const double p[3] = {east, north, up};
     return horizon_loss_db(at, sectors, p);
+}
+
+// NPD distance levels of a track's emission: sel_db[10], scaled_distance_m[10], tail_sel_db.
+int qmo_npd_distance_levels(const qm_aircraft_class* classes, const qm_track* track, double* out /* 21 */) {
+    const Class c = class_of(classes[track->aircraft_class]);
+    Emission em{&c, (int)track->departure, (int)track->power_row, (double)track->power_weight,
+                speed_correction_db(c.v_ref_kt, (double)track->speed_kt), c.installation};
+    const DistanceLevels lv = npd_distance_levels(em);
+    for (int k = 0; k < ND; ++k) {
+        out[k] = lv.sel_db[k];
+        out[ND + k] = lv.scaled_distance_m[k];
+    }
+    out[2 * ND] = lv.tail_sel_db;
+    return 0;
+}
+
+// BoxSums over `n` pieces: piece i's emission is `tracks[i]`'s, its period weights
+// period_weights[3 i..], its ends ends[6 i..] (start xyz, end xyz). `values` receives, in order:
+// levels_db[3][10], tail_levels_db[3], scaled_distance_m[10], centroid[3], axis_rad, gradient,
+// gradient_spread, piece_length_m[3], installation_shares[3][2][3], average_length_m (= 71
+// doubles). Returns 0, or 1 for a box without energy.
+int qmo_box_values(const qm_aircraft_class* classes, const qm_track* tracks, const double* period_weights,
+                   const double* ends, size_t n, double* values) {
+    BoxSums sums;
+    for (size_t i = 0; i < n; ++i) {
+        const Class c = class_of(classes[tracks[i].aircraft_class]);
+        Emission em{&c, (int)tracks[i].departure, (int)tracks[i].power_row, (double)tracks[i].power_weight,
+                    speed_correction_db(c.v_ref_kt, (double)tracks[i].speed_kt), c.installation};
+        sums.add(npd_distance_levels(em), c.installation, period_weights + 3 * i, ends + 6 * i, ends + 6 * i + 3);
+    }
+    BoxValues v;
+    if (!sums.values(v)) return 1;
+    double* o = values;
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < ND; ++k) *o++ = v.levels_db[p][k];
+    for (int p = 0; p < 3; ++p) *o++ = v.tail_levels_db[p];
+    for (int k = 0; k < ND; ++k) *o++ = v.scaled_distance_m[k];
+    for (int a = 0; a < 3; ++a) *o++ = v.centroid_m[a];
+    *o++ = v.axis_rad;
+    *o++ = v.gradient;
+    *o++ = v.gradient_spread;
+    for (int p = 0; p < 3; ++p) *o++ = v.piece_length_m[p];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 3; ++k) *o++ = v.installation_shares[p][q][k];
+    *o++ = average_length_m(v.levels_db, v.piece_length_m);
+    return 0;
+}
+
+// box_sel_at_receiver of box values (the 71 doubles above) with the centroid relative to the
+// receiver replaced by `centroid` (xyz); `horizon` null for unscreened (else QM_HORIZON_FLOATS,
+// the box screened). Out: sel energy per period and the received installation fractions [3][3].
+int qmo_box_sel(const double* values, const double* centroid, const float* horizon, double* sel,
+                double* fractions) {
+    BoxAt b;
+    const double* v = values;
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < ND; ++k) b.levels_db[p][k] = *v++;
+    for (int p = 0; p < 3; ++p) b.tail_levels_db[p] = *v++;
+    for (int k = 0; k < ND; ++k) b.lg_scaled_distance[k] = std::log10(*v++);
+    v += 3;
+    const double axis = *v++;
+    b.sin_axis = std::sin(axis);
+    b.cos_axis = std::cos(axis);
+    b.gradient = *v++;
+    b.gradient_spread = *v++;
+    for (int p = 0; p < 3; ++p) b.piece_length_m[p] = *v++;
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 3; ++k) b.installation_shares[p][q][k] = *v++;
+    for (int a = 0; a < 3; ++a) b.centroid_m[a] = centroid[a];
+    b.screened = horizon != nullptr;
+    double f[3][3];
+    box_sel(b, average_length_m(b.levels_db, b.piece_length_m), horizon, sel, f);
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 3; ++k) fractions[3 * p + k] = f[p][k];
+    return 0;
+}
+
+// aircraft.cuh flight_sums (tile_flight_passes) over the host's records in FP64: per segment its
+// boxes within reach, weighted, into its sum slot (QM_FLIGHT_SUMS doubles, zeroed first). Returns
+// -1 for a record out of range.
+int qmo_flight_sums(const qm_flight_box* boxes, size_t n_boxes, const qm_flight_point* points, size_t n_points,
+                    const float* horizons, size_t n_horizon_floats, const qm_flight_segment* segments,
+                    size_t n_segments, double* sums, size_t n_sums) {
+    for (size_t i = 0; i < n_sums * QM_FLIGHT_SUMS; ++i) sums[i] = 0.0;
+    for (size_t g = 0; g < n_segments; ++g) {
+        const qm_flight_segment& sg = segments[g];
+        if (sg.point >= n_points || sg.sum >= n_sums || (size_t)sg.first + sg.count > n_boxes) return -1;
+        const qm_flight_point& pt = points[sg.point];
+        const float* horizon = nullptr;
+        if (pt.horizons != QM_NO_HORIZONS) {
+            if (pt.horizons + QM_HORIZON_FLOATS > n_horizon_floats) return -1;
+            horizon = horizons + pt.horizons;
+        }
+        const double reach = std::min(sg.within_m, QM_AIRCRAFT_REACH_M);
+        double* at = sums + (size_t)sg.sum * QM_FLIGHT_SUMS;
+        double* class_energy = at + 3;
+        double* class_passing = class_energy + 3 * QM_PASS_CLASSES;
+        for (uint32_t k = 0; k < sg.count; ++k) {
+            const qm_flight_box& r = boxes[sg.first + k];
+            const double east = r.east_m - pt.x, north = r.north_m - pt.y;
+            const double distance = std::hypot(east, north);
+            if (distance > reach) continue;
+            const double share = flight_weight(sg.weight, distance, (double)sg.coarse_m);
+            if (share <= 0.0) continue;
+            const BoxAt b = box_at(r, pt);
+            double sel[3], f[3][3];
+            box_sel(b, r.average_length_m, horizon, sel, f);
+            const double impedance = r.helicopter ? 1.0 : (double)pt.impedance_gain;
+            for (int p = 0; p < 3; ++p) {
+                const double e = share * sel[p] * impedance / (QM_PERIOD_HOURS[p] * 3600.0);
+                const double passing = share * r.passing[p];
+                at[p] += e;
+                if (e > 0.0 && passing > 0.0) {
+                    const int c = p * QM_PASS_CLASSES + pass_class(e, passing);
+                    class_energy[c] += e;
+                    class_passing[c] += passing;
+                }
+            }
+        }
+    }
+    return 0;
 }
 
 }  // extern "C"

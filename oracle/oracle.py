@@ -95,6 +95,11 @@ def _declare(L):
     L.qmo_installation_gain.restype = C.c_double
     L.qmo_closest_points.argtypes = [D, D, D]
     L.qmo_segment_levels.argtypes = [C.c_void_p, C.c_void_p, D, D, D, D]
+    L.qmo_npd_distance_levels.argtypes = [C.c_void_p, C.c_void_p, D]
+    L.qmo_box_values.argtypes = [C.c_void_p, C.c_void_p, D, D, C.c_size_t, D]
+    L.qmo_box_sel.argtypes = [D, D, F32, D, D]
+    L.qmo_flight_sums.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, F32, C.c_size_t, C.c_void_p,
+                                  C.c_size_t, D, C.c_size_t]
     L.qmo_aircraft_points.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, D, C.c_size_t,
                                       C.c_double, C.c_double, D, U32, U32, U32, D, C.c_void_p, U64, C.c_int]
     L.qmo_path_difference_loss.argtypes = [C.c_double]
@@ -508,3 +513,68 @@ def horizons(scene, x, y, altitude_m, own_footprint=0):
     failed = lib().qmo_horizons(C.byref(scene.terrain.c), C.byref(scene.obstacles.c), float(x), float(y),
                                 float(altitude_m), int(own_footprint), _p(out, F32))
     return out, failed
+
+
+BOX_VALUE_FIELDS = (("levels_db", (3, 10)), ("tail_levels_db", (3,)), ("scaled_distance_m", (10,)),
+                    ("centroid_m", (3,)), ("axis_rad", ()), ("gradient", ()), ("gradient_spread", ()),
+                    ("piece_length_m", (3,)), ("installation_shares", (3, 2, 3)), ("average_length_m", ()))
+
+
+def npd_distance_levels(classes, track):
+    """segment.rs npd_distance_levels of a track's emission: (sel_db[10], scaled_distance_m[10],
+    tail_sel_db)."""
+    classes = np.ascontiguousarray(classes, abi.AIRCRAFT_CLASS_DTYPE)
+    track = np.ascontiguousarray(np.atleast_1d(track), abi.TRACK_DTYPE)
+    out = np.zeros(21)
+    lib().qmo_npd_distance_levels(classes.ctypes.data, track.ctypes.data, _p(out, D))
+    return out[:10], out[10:20], out[20]
+
+
+def box_values(classes, tracks, period_weights, ends):
+    """BoxSums of pieces (track i's emission, its period weights [3], its ends [start xyz, end
+    xyz]): a dict of the box values and the average length, or None for a box without energy."""
+    classes = np.ascontiguousarray(classes, abi.AIRCRAFT_CLASS_DTYPE)
+    tracks = np.ascontiguousarray(tracks, abi.TRACK_DTYPE)
+    pw = np.ascontiguousarray(period_weights, np.float64).reshape(-1, 3)
+    ends = np.ascontiguousarray(ends, np.float64).reshape(-1, 6)
+    assert pw.shape[0] == ends.shape[0] == tracks.size
+    out = np.zeros(71)
+    if lib().qmo_box_values(classes.ctypes.data, tracks.ctypes.data, _p(pw, D), _p(ends, D), tracks.size,
+                            _p(out, D)):
+        return None
+    values, at = {}, 0
+    for name, shape in BOX_VALUE_FIELDS:
+        n = int(np.prod(shape)) if shape else 1
+        values[name] = out[at:at + n].reshape(shape) if shape else float(out[at])
+        at += n
+    assert at == 71
+    return values
+
+
+def _pack_values(values):
+    return np.concatenate([np.asarray(values[name], np.float64).reshape(-1) for name, _ in BOX_VALUE_FIELDS])
+
+
+def box_sel(values, centroid, horizon=None):
+    """box_sel_at_receiver of box values with the centroid at `centroid` (metres from the
+    receiver): SEL energy per period and the installation fractions [3][3]."""
+    v = _pack_values(values)
+    c = np.ascontiguousarray(centroid, np.float64)
+    sel, frac = np.zeros(3), np.zeros(9)
+    h = None if horizon is None else np.ascontiguousarray(horizon, np.float32)
+    lib().qmo_box_sel(_p(v, D), _p(c, D), None if h is None else _p(h, F32), _p(sel, D), _p(frac, D))
+    return sel, frac.reshape(3, 3)
+
+
+def flight_sums(boxes, points, horizons, segments, n_sums):
+    """The card's flight_sums over host records in FP64: [n_sums, FLIGHT_SUMS]."""
+    boxes = np.ascontiguousarray(boxes, abi.FLIGHT_BOX_DTYPE)
+    points = np.ascontiguousarray(points, abi.FLIGHT_POINT_DTYPE)
+    segments = np.ascontiguousarray(segments, abi.FLIGHT_SEGMENT_DTYPE)
+    h = np.ascontiguousarray(horizons if horizons is not None else np.zeros(0), np.float32).reshape(-1)
+    out = np.zeros((n_sums, abi.FLIGHT_SUMS))
+    rc = lib().qmo_flight_sums(boxes.ctypes.data, boxes.size, points.ctypes.data, points.size, _p(h, F32), h.size,
+                               segments.ctypes.data, segments.size, _p(out, D), n_sums)
+    if rc:
+        raise ValueError("a flight record out of range")
+    return out

@@ -266,3 +266,43 @@ TRACK_DTYPE = np.dtype(
 )
 assert AIRCRAFT_CLASS_DTYPE.itemsize == 2 * 2 * POWER_ROWS * NPD_DISTANCES * 8 + 24
 assert TRACK_DTYPE.itemsize == 40
+
+# Aircraft boxes at the painter's points (qm_aircraft.h qm_flight_box / qm_flight_point /
+# qm_flight_segment; the reference card's FlightBox / FlightPoint / FlightSegment).
+FLIGHT_SUMS = 3 + 2 * 3 * PASS_CLASSES
+FLIGHT_WEIGHT_ONE, FLIGHT_WEIGHT_FAR, FLIGHT_WEIGHT_NEAR = 0, 1, 2
+AIRCRAFT_REACH_M = 16000.0
+NO_HORIZONS = 0xFFFFFFFFFFFFFFFF
+FLIGHT_BOX_DTYPE = np.dtype(
+    [
+        ("east_m", "<f8"),
+        ("north_m", "<f8"),
+        ("centroid_altitude_m", "<f8"),
+        ("sin_axis", "<f8"),
+        ("cos_axis", "<f8"),
+        ("gradient", "<f8"),
+        ("gradient_spread", "<f8"),
+        ("average_length_m", "<f8"),
+        ("passing", "<f8", (3,)),
+        ("piece_length_m", "<f4", (3,)),
+        ("levels_db", "<f4", (3, NPD_DISTANCES)),
+        ("tail_levels_db", "<f4", (3,)),
+        ("lg_scaled_distance", "<f4", (NPD_DISTANCES,)),
+        ("installation_shares", "<f4", (3, 2, 3)),
+        ("screened", "<u4"),
+        ("helicopter", "<u4"),
+        ("silent", "<u4", (3,)),
+        ("pad", "<u4"),
+    ]
+)
+FLIGHT_POINT_DTYPE = np.dtype(
+    [("x", "<f8"), ("y", "<f8"), ("altitude_m", "<f8"), ("horizons", "<u8"), ("impedance_gain", "<f4"),
+     ("pad", "<u4")]
+)
+FLIGHT_SEGMENT_DTYPE = np.dtype(
+    [("within_m", "<f8"), ("point", "<u4"), ("sum", "<u4"), ("first", "<u4"), ("count", "<u4"), ("weight", "<u4"),
+     ("coarse_m", "<f4")]
+)
+assert FLIGHT_BOX_DTYPE.itemsize == 368
+assert FLIGHT_POINT_DTYPE.itemsize == 40
+assert FLIGHT_SEGMENT_DTYPE.itemsize == 32

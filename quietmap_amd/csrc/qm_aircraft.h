@@ -90,6 +90,67 @@ typedef struct qm_track {
     uint32_t flights[3];   /* flights a day: day, evening, night */
 } qm_track;
 
+/* Aircraft boxes at the painter's points: the records the reference's host writes for its card's
+ * flight_sums (paint-gpu/kernels/aircraft.cuh:14-50, filled by paint-gpu/src/aircraft.rs
+ * DeviceFlightBoxes::upload and Gpu::flight_sums), field for field. A box (a tile's
+ * tiles::aircraft::AircraftBox in the square's frame): its centroid east and north (frame
+ * metres) and altitude, its axis, gradient and the gradients' spread, the average piece's length
+ * (physics/src/doc29/boxes.rs average_length_m), the share of each period its flights are in it
+ * (popup/src/aircraft/boxes.rs box_passing), per period the pieces' mean horizontal length, the
+ * summed SEL at the ten NPD distances (dB, -inf silent) and at the tail anchor, lg d_lambda at the
+ * NPD distances, the installation shares [period][at 1,000 ft, at the tail anchor][wing, fuselage,
+ * propeller], whether it is screened (centroid below QM_SCREENING_CEILING_M over its ground), a
+ * helicopter box (no receiver impedance) and each period's silence. */
+#define QM_FLIGHT_SUMS (3 + 2 * 3 * QM_PASS_CLASSES)
+#define QM_FLIGHT_WEIGHT_ONE 0u
+#define QM_FLIGHT_WEIGHT_FAR 1u
+#define QM_FLIGHT_WEIGHT_NEAR 2u
+#define QM_AIRCRAFT_REACH_M 16000.0
+/* A point whose `horizons` is QM_NO_HORIZONS is heard unscreened. */
+#define QM_NO_HORIZONS UINT64_MAX
+
+typedef struct qm_flight_box {
+    double east_m, north_m;
+    double centroid_altitude_m;
+    double sin_axis, cos_axis;
+    double gradient, gradient_spread;
+    double average_length_m;
+    double passing[3];
+    float piece_length_m[3];
+    float levels_db[3][QM_NPD_DISTANCES];
+    float tail_levels_db[3];
+    float lg_scaled_distance[QM_NPD_DISTANCES];
+    float installation_shares[3][2][3];
+    uint32_t screened;
+    uint32_t helicopter;
+    uint32_t silent[3];
+} qm_flight_box;
+
+/* A point: where it stands (frame metres) and its altitude, the offset of its horizon record
+ * (QM_HORIZON_FLOATS floats) in the horizon array, and the NPD levels' impedance gain at its
+ * elevation (energy, physics/src/doc29/atmosphere.rs receiver_impedance_db). */
+typedef struct qm_flight_point {
+    double x, y;
+    double altitude_m;
+    uint64_t horizons;
+    float impedance_gain;
+} qm_flight_point;
+
+/* One point's boxes of one tile, file and band: `count` boxes from `first`, heard within
+ * `within_m` (and QM_AIRCRAFT_REACH_M), each weighted by `weight` (QM_FLIGHT_WEIGHT_*, the
+ * smoothstep from 1.5 to 3 `coarse_m`) at its horizontal distance, added to sum slot `sum`
+ * (QM_FLIGHT_SUMS doubles: energy per period, then per period the pass classes' energies, then
+ * their passing shares). */
+typedef struct qm_flight_segment {
+    double within_m;
+    uint32_t point;
+    uint32_t sum;
+    uint32_t first;
+    uint32_t count;
+    uint32_t weight;
+    float coarse_m;
+} qm_flight_segment;
+
 #ifdef __cplusplus
 }
 #endif

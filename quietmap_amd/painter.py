@@ -40,6 +40,11 @@ EXPORTS = (
     "qm_pixel_loudness",
     "qm_kurze_table",
     "qm_horizons",
+    "qm_flight_boxes_upload",
+    "qm_flight_boxes_count",
+    "qm_flight_boxes_free",
+    "qm_flight_sums",
+    "qm_flight_sums_device",
     "qm_mlp_new",
     "qm_mlp_params",
     "qm_mlp_free",
@@ -93,6 +98,13 @@ def library():
         lib.qm_kurze_table.argtypes = [vp]
         lib.qm_kurze_table.restype = None
         lib.qm_horizons.argtypes = [vp, vp, vp, vp, sz, vp, vp, cp, sz]
+        lib.qm_flight_boxes_upload.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
+        lib.qm_flight_boxes_count.argtypes = [vp]
+        lib.qm_flight_boxes_count.restype = sz
+        lib.qm_flight_boxes_free.argtypes = [vp]
+        lib.qm_flight_boxes_free.restype = None
+        lib.qm_flight_sums.argtypes = [vp, vp, vp, sz, vp, sz, vp, sz, vp, sz, cp, sz]
+        lib.qm_flight_sums_device.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, cp, sz]
         lib.qm_mlp_new.argtypes = [vp, vp, sz, C.POINTER(vp), cp, sz]
         lib.qm_mlp_params.argtypes = [vp, vp, cp, sz]
         lib.qm_mlp_free.argtypes = [vp]
@@ -136,6 +148,24 @@ class Square:
     def close(self):
         if self.handle:
             library().qm_square_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FlightBoxes:
+    """A square's aircraft boxes on the device (DeviceFlightBoxes)."""
+
+    def __init__(self, handle, n):
+        self.handle, self.n = handle, n
+
+    def close(self):
+        if self.handle:
+            library().qm_flight_boxes_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -308,6 +338,33 @@ class Gpu:
                                    err, 512)
         _check(rc, err)
         return out, failed
+
+    def flight_boxes(self, boxes):
+        """Uploads aircraft box records (abi.FLIGHT_BOX_DTYPE): a FlightBoxes handle."""
+        boxes = np.ascontiguousarray(boxes, abi.FLIGHT_BOX_DTYPE)
+        err = C.create_string_buffer(512)
+        handle = C.c_void_p()
+        _check(library().qm_flight_boxes_upload(self.handle, _ptr(boxes), boxes.size, C.byref(handle), err, 512), err)
+        return FlightBoxes(handle, boxes.size)
+
+    def flight_sums(self, boxes, points, horizons, segments, n_sums):
+        """Per sum slot the boxes' energy per period and pass classes [n_sums, FLIGHT_SUMS]."""
+        pts = np.ascontiguousarray(points, abi.FLIGHT_POINT_DTYPE)
+        segs = np.ascontiguousarray(segments, abi.FLIGHT_SEGMENT_DTYPE)
+        h = None if horizons is None else np.ascontiguousarray(horizons, np.float32).reshape(-1)
+        out = np.zeros((n_sums, abi.FLIGHT_SUMS))
+        err = C.create_string_buffer(512)
+        rc = library().qm_flight_sums(self.handle, boxes.handle, _ptr(pts), pts.size, _ptr(h), 0 if h is None else h.size,
+                                      _ptr(segs), segs.size, _ptr(out), n_sums, err, 512)
+        _check(rc, err)
+        return out
+
+    def flight_sums_device(self, boxes, d_points: int, d_horizons: int, d_segments: int, n_segments: int,
+                           d_sums: int, stream: int = 0):
+        err = C.create_string_buffer(512)
+        rc = library().qm_flight_sums_device(self.handle, boxes.handle, d_points, d_horizons or None, d_segments,
+                                             n_segments, d_sums, stream or None, err, 512)
+        _check(rc, err)
 
     def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False):
         """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands."""

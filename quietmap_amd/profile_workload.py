@@ -7,7 +7,10 @@ in one paint_kernel launch (the defaults are the benchmark's step), `surrogate` 
 window with the trained network (quietmap_amd/surrogate_params.npy), `loudness` runs
 zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
 `--aircraft-scale` (`passes` with its 48 pass classes per period), `pixels` runs the pixel
-loudness kernels (mixtures_kernel, ndens_kernel) over `--pixels` synthetic packed pixels, and
+loudness kernels (mixtures_kernel, ndens_kernel) over `--pixels` synthetic packed pixels,
+`flights` sums `--flight-boxes` synthetic aircraft boxes over 20 km at `--flight-points` points
+(every point hearing all 16 tiles: flight_sums_kernel), `horizons` builds the horizons of
+`--horizon-points` receivers of the tile-scale scene at `--scale` 0.25 (horizons_kernel), and
 `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
 line when it ends.
@@ -34,6 +37,9 @@ def main():
     ap.add_argument("--aircraft-scale", type=float, default=0.05)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--pixels", type=int, default=4096)
+    ap.add_argument("--flight-boxes", type=int, default=20000)
+    ap.add_argument("--flight-points", type=int, default=4096)
+    ap.add_argument("--horizon-points", type=int, default=4096)
     args = ap.parse_args()
     kernels = args.kernels.split(",")
     gpu = Gpu(0)
@@ -100,6 +106,28 @@ def main():
         nd = gpu.pixel_loudness(packed, offsets)
         print(f"pixels: {args.pixels} pixels, place Nden mean {float(nd[:, 6].mean()):.2f} sone, "
               f"{time.time() - t:.2f} s", flush=True)
+    if "flights" in kernels:
+        from . import flights
+
+        boxes = flights.synthetic_boxes(seed=31, count=args.flight_boxes, extent_m=20000.0)
+        side = int(round(args.flight_points ** 0.5))
+        jj, ii = np.mgrid[0:side, 0:side]
+        xy = np.stack([(ii.reshape(-1) + 0.5) * 20000.0 / side, (jj.reshape(-1) + 0.5) * 20000.0 / side], 1)
+        boxes, fpoints, segments, n_sums = flights.flight_requests(boxes, xy, extent_m=20000.0)
+        handle = gpu.flight_boxes(boxes)
+        t = time.time()
+        sums = gpu.flight_sums(handle, fpoints, None, segments, n_sums)
+        print(f"flights: {boxes.size} boxes, {fpoints.size} points, {segments.size} segments, "
+              f"{int(segments['count'].astype(np.int64).sum())} box reads, heard energy {float(sums[:, 0].mean()):.3e}, "
+              f"{time.time() - t:.2f} s", flush=True)
+        handle.close()
+    if "horizons" in kernels:
+        sc = scenes.tile_scene(scale=0.25)
+        square = gpu.upload(sc.scene, sc.sources)
+        pts = sc.points[np.random.default_rng(5).choice(sc.points.size, args.horizon_points, replace=False)]
+        t = time.time()
+        rec, failed = gpu.horizons(square, pts)
+        print(f"horizons: {pts.size} receivers, {int((failed != 0).sum())} failed, {time.time() - t:.2f} s", flush=True)
     if "copy" in kernels:
         nbytes = args.copy_mib << 20
         a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)

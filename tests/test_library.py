@@ -45,3 +45,38 @@ def test_screening_struct_matches_the_header():
 
     assert C.sizeof(abi.AircraftScreening) == 32
     assert abi.HORIZON_FLOATS == 3746 and abi.TERRAIN_HORIZON_FLOATS == 417
+
+
+def test_flight_records_match_the_header(tmp_path):
+    """abi's flight record dtypes have the C structs' sizes and field offsets (qm_aircraft.h)."""
+    import shutil
+    import subprocess
+
+    from quietmap_amd import abi
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    header = os.path.join(os.path.dirname(painter.HERE), "quietmap_amd", "csrc", "qm_aircraft.h")
+    checks = {
+        "qm_flight_box": (abi.FLIGHT_BOX_DTYPE, ["east_m", "passing", "piece_length_m", "levels_db", "tail_levels_db",
+                                                 "lg_scaled_distance", "installation_shares", "screened", "silent"]),
+        "qm_flight_point": (abi.FLIGHT_POINT_DTYPE, ["x", "altitude_m", "horizons", "impedance_gain"]),
+        "qm_flight_segment": (abi.FLIGHT_SEGMENT_DTYPE, ["within_m", "point", "sum", "first", "count", "weight",
+                                                         "coarse_m"]),
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{header}"', "int main(void) {"]
+    for struct, (_, fields) in checks.items():
+        lines.append(f'printf("%zu\\n", sizeof({struct}));')
+        for f in fields:
+            lines.append(f'printf("%zu\\n", offsetof({struct}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    values = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = []
+    for struct, (dtype, fields) in checks.items():
+        want.append(dtype.itemsize)
+        want.extend(dtype.fields[f][1] for f in fields)
+    assert values == want
