@@ -80,9 +80,9 @@ int qmo_ray_terms(size_t n, double horizontal_m, const double* t, const double* 
 int qmo_vertical_path(size_t n, double horizontal_m, const double* t, const double* ground_m, const double* ground_factor,
                       size_t n_crossings, const double* crossing_t, const double* crossing_height,
                       const uint8_t* crossing_building, const uint8_t* crossing_roof, const uint64_t* crossing_footprint,
-                      double source_height_m, double platform_half_width_m, uint64_t own_footprint, size_t capacity,
-                      double* d, double* z, double* g, size_t* n_out, double* tops, size_t* n_tops, double* crests,
-                      size_t* n_crests) {
+                      double source_height_m, double platform_half_width_m, uint64_t own_footprint, int has_source_ground,
+                      double source_ground_m, size_t capacity, double* d, double* z, double* g, size_t* n_out,
+                      double* tops, size_t* n_tops, double* crests, size_t* n_crests) {
     Profile profile;
     profile.clear(horizontal_m);
     for (size_t i = 0; i < n; ++i) profile.push(t[i], ground_m[i], ground_factor[i], 0.0, 0.0);
@@ -90,7 +90,8 @@ int qmo_vertical_path(size_t n, double horizontal_m, const double* t, const doub
     for (size_t i = 0; i < n_crossings; ++i)
         crossings[i] = Crossing{crossing_t[i], crossing_height[i], crossing_building[i] != 0, crossing_roof[i] != 0,
                                 crossing_footprint[i]};
-    RayEnds ends{source_height_m, false, 0.0, 0.0, false, 0.0, platform_half_width_m, own_footprint};
+    RayEnds ends{source_height_m, has_source_ground != 0, source_ground_m, 0.0, false, 0.0, platform_half_width_m,
+                 own_footprint};
     RayScratch s;
     fill_path(s, profile, crossings, ends);
     *n_out = s.distance_m.size();

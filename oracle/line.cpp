@@ -249,7 +249,7 @@ void skyline_arcs(const qm_obstacles& O, const double origin[2], double lo_rad, 
     }
 }
 
-// popup/src/evaluate.rs:171-267 source_rays for a line piece (no deck): every quadrature node a ray
+// popup/src/evaluate.rs:171-267 source_rays for a line piece: every quadrature node a ray
 // from its point to the receiver, weighted by its dphi times the piece's divergence factor, the
 // wall search skipped inside the skyline's clear gaps; received as add_ray (evaluate.rs:119-152).
 int evaluate_line_pair(const Scene& S, const Receiver& receiver, const qm_line_piece& piece, PairScratch& s,
@@ -261,15 +261,18 @@ int evaluate_line_pair(const Scene& S, const Receiver& receiver, const qm_line_p
     }
     for (int st = 0; st < 2; ++st)
         for (int b = 0; b < BANDS; ++b) out.boundary[st][b] = 0.0;
-    // candidate.traced_ground_m: the ground at the piece's ends.
+    // candidate.traced_ground_m (candidates.rs:151-158): the deck at the piece's ends, else the
+    // terrain there.
     GroundSample ga, gb;
     int failed = ground_at(S.terrain, piece.ax_m, piece.ay_m, ga);
     if (!failed) failed = ground_at(S.terrain, piece.bx_m, piece.by_m, gb);
     if (failed) return failed;
+    const double ground_a = piece.on_deck ? (double)piece.deck_a_m : ga.height_m;
+    const double ground_b = piece.on_deck ? (double)piece.deck_b_m : gb.height_m;
     const double start[3] = {piece.ax_m - receiver.x, piece.ay_m - receiver.y,
-                             ga.height_m + piece.height_m - receiver.altitude_m};
+                             ground_a + piece.height_m - receiver.altitude_m};
     const double end[3] = {piece.bx_m - receiver.x, piece.by_m - receiver.y,
-                           gb.height_m + piece.height_m - receiver.altitude_m};
+                           ground_b + piece.height_m - receiver.altitude_m};
     LineGeometry g;
     if (!line_geometry(start, end, g)) return 0;
     const double origin[2] = {receiver.x, receiver.y};
@@ -301,8 +304,10 @@ int evaluate_line_pair(const Scene& S, const Receiver& receiver, const qm_line_p
         for (int period = 0; period < PERIODS; ++period) p[period] = favourable_at(receiver.favourable, period, azimuth);
         RayEnds ends;
         ends.source_height_m = piece.height_m;
-        ends.has_source_ground = false;
-        ends.source_ground_m = 0.0;
+        // evaluate.rs:182-186 deck_at(fraction): a node on a deck stands on it.
+        ends.has_source_ground = piece.on_deck != 0;
+        ends.source_ground_m =
+            piece.on_deck ? (double)piece.deck_a_m + fraction * ((double)piece.deck_b_m - (double)piece.deck_a_m) : 0.0;
         ends.receiver_altitude_m = receiver.altitude_m;
         ends.has_source_ground_factor = false;
         ends.source_ground_factor = 0.0;

@@ -51,7 +51,8 @@ def _declare(L):
                                 C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                 C.c_uint64, D, D, D, D, D, D]
     L.qmo_vertical_path.argtypes = [C.c_size_t, C.c_double, D, D, D, C.c_size_t, D, D, U8, U8, U64, C.c_double,
-                                    C.c_double, C.c_uint64, C.c_size_t, D, D, D, SZ, D, SZ, D, SZ]
+                                    C.c_double, C.c_uint64, C.c_int, C.c_double, C.c_size_t, D, D, D, SZ, D, SZ, D,
+                                    SZ]
     L.qmo_ground_at.argtypes = [C.c_void_p, C.c_double, C.c_double, D]
     L.qmo_fill_profile.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_size_t,
                                    D, D, D, D, D, SZ, D]
@@ -256,7 +257,8 @@ def ray_terms(profile, crossings, ends, favourable, alpha):
                 boundary_db=boundary.reshape(2, 8))
 
 
-def vertical_path(profile, crossings, source_height_m, platform_half_width_m=0.0, own_footprint=0, capacity=4096):
+def vertical_path(profile, crossings, source_height_m, platform_half_width_m=0.0, own_footprint=0, capacity=4096,
+                  source_ground_m=None):
     arr = {k: np.ascontiguousarray(profile[k], np.float64) for k in ("t", "ground_m", "ground_factor")}
     ct = np.array([c[0] for c in crossings], np.float64)
     ch = np.array([c[1] for c in crossings], np.float64)
@@ -268,7 +270,9 @@ def vertical_path(profile, crossings, source_height_m, platform_half_width_m=0.0
     n, nt, nc = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
     rc = lib().qmo_vertical_path(arr["t"].size, profile["horizontal_m"], _p(arr["t"], D), _p(arr["ground_m"], D),
                                  _p(arr["ground_factor"], D), ct.size, _p(ct, D), _p(ch, D), _p(cb, U8), _p(cr, U8),
-                                 _p(cf, U64), source_height_m, platform_half_width_m, own_footprint, capacity,
+                                 _p(cf, U64), source_height_m, platform_half_width_m, own_footprint,
+                                 int(source_ground_m is not None), 0.0 if source_ground_m is None else source_ground_m,
+                                 capacity,
                                  _p(d, D), _p(z, D), _p(g, D), C.byref(n), _p(tops, D), C.byref(nt), _p(crests, D),
                                  C.byref(nc))
     assert rc == 0

@@ -576,8 +576,8 @@ int evaluate_pair(const Scene& S, const Receiver& receiver, const qm_source& sou
     for (int period = 0; period < PERIODS; ++period) p[period] = favourable_at(receiver.favourable, period, azimuth);
     RayEnds ends;
     ends.source_height_m = source.height_m;
-    ends.has_source_ground = false;
-    ends.source_ground_m = 0.0;
+    ends.has_source_ground = source.on_deck != 0;  // evaluate.rs:193-196 a deck's ground (deck_at(0))
+    ends.source_ground_m = source.on_deck ? (double)source.deck_m : 0.0;
     ends.receiver_altitude_m = receiver.altitude_m;
     ends.has_source_ground_factor = source.ground_percent != QM_GROUND_FROM_TERRAIN;
     ends.source_ground_factor = (double)source.ground_percent / 100.0;
@@ -586,7 +586,8 @@ int evaluate_pair(const Scene& S, const Receiver& receiver, const qm_source& sou
     ray_terms(s.profile, s.crossings, ends, p, receiver.alpha, s.ray, s.terms, work);
     // evaluate.rs:195-203: the point divergence at the candidate's distance.
     const double horizontal = std::hypot(source.x_m - receiver.x, source.y_m - receiver.y);
-    const double ground0 = s.profile.ground_m[0];
+    // candidate.traced_ground_m (candidates.rs:151-158): the deck, else the terrain at the source.
+    const double ground0 = source.on_deck ? (double)source.deck_m : s.profile.ground_m[0];
     const double d = std::max(std::hypot(std::max(horizontal, (double)source.exclusion_radius_m),
                                          receiver.altitude_m - (ground0 + (double)source.height_m)),
                               1.0);

@@ -46,22 +46,25 @@ SOURCE_DTYPE = np.dtype(
         ("exclusion_radius_m", "<f4"),
         ("ground_percent", "u1"),
         ("layer", "u1"),
-        ("pad", "u1", (2,)),
+        ("on_deck", "u1"),
+        ("pad", "u1"),
         ("footprint_id", "<u8"),
         ("energy", "<f4", (PERIODS, BANDS)),
+        ("deck_m", "<f4"),
+        ("pad2", "<u4"),
     ]
 )
 POINT_DTYPE = np.dtype([("x_m", "<f8"), ("y_m", "<f8"), ("own_footprint", "<u8")])
 # qm_line_piece (qm_scene.h): a straight line-source piece and its emission per metre.
 LINE_PIECE_DTYPE = np.dtype([("ax_m", "<f8"), ("ay_m", "<f8"), ("bx_m", "<f8"), ("by_m", "<f8"),
-                             ("height_m", "<f4"), ("layer", "u1"), ("pad", "u1", 3),
-                             ("energy_per_m", "<f4", (3, 8))])
-assert LINE_PIECE_DTYPE.itemsize == 136
+                             ("height_m", "<f4"), ("layer", "u1"), ("on_deck", "u1"), ("pad", "u1", 2),
+                             ("energy_per_m", "<f4", (3, 8)), ("deck_a_m", "<f4"), ("deck_b_m", "<f4")])
+assert LINE_PIECE_DTYPE.itemsize == 144
 # qm_tile (qm_tiles.h): a tile's receivers (row-major nx x ny from receiver_first) and its sources.
 TILE_DTYPE = np.dtype([("receiver_first", "<u4"), ("nx", "<u4"), ("ny", "<u4"), ("source_first", "<u4"),
                        ("source_count", "<u4")])
 assert RUN_DTYPE.itemsize == 8 and OUTLINE_DTYPE.itemsize == 24
-assert SOURCE_DTYPE.itemsize == 136 and POINT_DTYPE.itemsize == 24
+assert SOURCE_DTYPE.itemsize == 144 and POINT_DTYPE.itemsize == 24
 
 
 def _ptr(array, ctype):

@@ -46,10 +46,13 @@ __global__ __launch_bounds__(256) void line_nodes_kernel(Scene sc, const qm_rece
         failed[i] = f;
         return;
     }
+    // On a bridge, the deck at the ends.
+    const double ground_a = piece.on_deck ? (double)piece.deck_a_m : (double)ga.z;
+    const double ground_b = piece.on_deck ? (double)piece.deck_b_m : (double)gb.z;
     const double start[3] = {piece.ax_m - rcv.x_m, piece.ay_m - rcv.y_m,
-                             (double)ga.z + piece.height_m - (double)rcv.altitude_m};
+                             ground_a + piece.height_m - (double)rcv.altitude_m};
     const double end[3] = {piece.bx_m - rcv.x_m, piece.by_m - rcv.y_m,
-                           (double)gb.z + piece.height_m - (double)rcv.altitude_m};
+                           ground_b + piece.height_m - (double)rcv.altitude_m};
     LineGeometry g;
     if (!line_geometry(start, end, g)) return;
     int k = 0;
@@ -86,6 +89,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     src.exclusion_radius_m = 0.0f;
     src.ground_percent = QM_GROUND_FROM_TERRAIN;
     src.layer = piece.layer;
+    src.on_deck = piece.on_deck;
+    src.deck_m = piece.deck_a_m + node.along_m * (piece.deck_b_m - piece.deck_a_m);
     src.footprint_id = 0;
     for (int q = 0; q < PERIODS; ++q)
         for (int b = 0; b < BANDS; ++b) src.energy[q][b] = piece.energy_per_m[q][b];

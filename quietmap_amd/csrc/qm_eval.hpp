@@ -388,7 +388,9 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         out.failed = failed;
         return;
     }
-    const float source_ground = s0.z;
+    // The source stands on a bridge's deck where it has one, else on the terrain (ray.rs:133-134,
+    // 259-268 source_ground_m).
+    const float source_ground = src.on_deck ? src.deck_m : s0.z;
     const float source_alt = source_ground + src.height_m;
     const float receiver_alt = rcv.altitude_m;
     const float platform = src.platform_half_width_m;
@@ -504,14 +506,15 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     float next_roof = nroofs > 0 ? w.roofs[0].x0 : __builtin_inff();
     float covered_to = 0.0f;
 
-    // Vertex 0: the source's ground.
+    // Vertex 0: the source's ground (cut to a deck under it within the platform).
     Profile P;
-    P.z0 = s0.z;
-    P.head = Vertex{0.0f, s0.z, s0.g, 0.0, 0.0, 0.0};
+    const float ground0 = 0.0f < platform ? fminf(s0.z, source_ground) : s0.z;
+    P.z0 = ground0;
+    P.head = Vertex{0.0f, ground0, s0.g, 0.0, 0.0, 0.0};
     P.roof_near = P.head;
     P.roof_x1 = P.roof_top1 = 0.0f;
     foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
-    crest_vertex(w, P, cg, tp, n, S, R, H, 0.0f, s0.z, failed);
+    crest_vertex(w, P, cg, tp, n, S, R, H, 0.0f, ground0, failed);
 
     // The walk's lattice: the coarse one, or with FINE each stretch's in turn.
     Lattice WL = L;
@@ -641,7 +644,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     for (int q = 0; q < PERIODS; ++q) p[q] = favourable_at(rcv.favourable[q], azimuth);
     // evaluate.rs:195-203: the point divergence.
     const float hd = fmaxf((float)length_d, src.exclusion_radius_m);
-    const float dz = receiver_alt - (s0.z + src.height_m);
+    const float dz = receiver_alt - source_alt;
     const float d = fmaxf(sqrtf(hd * hd + dz * dz), 1.0f);
     const float weight = opt.weight >= 0.0f ? opt.weight : 1.0f / (d * d * 12.589254117941673f);
     float fol_db[2][BANDS];

@@ -38,7 +38,8 @@ QM_DEV SurrogatePath surrogate_features(const Scene& sc, const qm_receiver& rcv,
                                         float f[FEATURES]) {
     SurrogatePath out{rcv.failed, false};
     for (int i = 0; i < FEATURES; ++i) f[i] = 0.0f;
-    if (rcv.failed) return out;
+    // A source on a bridge's deck takes the exact path: the network knows sources on the terrain.
+    if (rcv.failed || src.on_deck) return out;
     const qm_terrain& T = sc.terrain;
     const double sx = src.x_m, sy = src.y_m, rx = rcv.x_m, ry = rcv.y_m;
     if (!isfinite(sx) || !isfinite(sy) || !isfinite(rx) || !isfinite(ry)) {

@@ -106,7 +106,11 @@ typedef struct qm_weather {
     const uint8_t* nodes; /* rows*columns*64 */
 } qm_weather;
 
-/* A point source: the candidate's place and its attribute (popup/src/candidates.rs:35-49). */
+/* A point source: the candidate's place and its attribute (popup/src/candidates.rs:35-49). A
+ * source `on_deck` stands on a bridge's deck at `deck_m` (the candidate's ground0, the same datum
+ * as the terrain) instead of the terrain under it (physics/src/ray.rs RayEnds::source_ground_m:
+ * its altitude is the deck's plus its height, and the terrain within its platform is cut to the
+ * deck). */
 typedef struct qm_source {
     double x_m, y_m;
     float height_m;
@@ -114,20 +118,27 @@ typedef struct qm_source {
     float exclusion_radius_m;
     uint8_t ground_percent; /* QM_GROUND_FROM_TERRAIN: the terrain's G under the source */
     uint8_t layer;
-    uint8_t pad[2];
+    uint8_t on_deck;
+    uint8_t pad;
     uint64_t footprint_id;
     float energy[QM_PERIODS][QM_BANDS]; /* A-weighted sound energy per period and band */
+    float deck_m;
+    uint32_t pad2;
 } qm_source;
 
 /* A straight line-source piece: a line candidate's ends and attribute (popup/src/candidates.rs:
  * 35-49 with `line` set; physics/src/line.rs integrates it). Ends in frame metres, the source's
- * height above the ground under it, the emission per metre of the piece. */
+ * height above the ground under it, the emission per metre of the piece. A piece `on_deck` runs
+ * on a bridge's deck from `deck_a_m` under its first end to `deck_b_m` under its second (the
+ * candidate's ground0 and ground1), each node standing on the deck between them. */
 typedef struct qm_line_piece {
     double ax_m, ay_m, bx_m, by_m;
     float height_m;
     uint8_t layer;
-    uint8_t pad[3];
+    uint8_t on_deck;
+    uint8_t pad[2];
     float energy_per_m[QM_PERIODS][QM_BANDS]; /* A-weighted sound energy per metre */
+    float deck_a_m, deck_b_m;
 } qm_line_piece;
 
 /* A receiver point (paint/src/exact.rs:18-34): where it stands and the building it stands in. */

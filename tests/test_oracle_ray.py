@@ -35,6 +35,20 @@ def test_platform_clamps_near_terrain():
     assert len(path["crests"]) == 6 and len(path["tops"]) == 0
 
 
+def test_a_source_on_a_deck_stands_on_it_above_the_valley():
+    """ray_tests.rs:82-104: a deck at 150 m leaves the valley in the profile (the clamp takes the
+    lower of the two) and the ray runs from the deck plus the source's height."""
+    path = oracle.vertical_path(profile(), [], 4.0, platform_half_width_m=30.0, own_footprint=1, source_ground_m=150.0)
+    np.testing.assert_allclose(path["z"], [100.0, 102.0, 105.0, 101.0, 103.0, 100.0], atol=1e-9)
+    ends = dict(source_height_m=4.0, receiver_altitude_m=104.0, source_ground_m=150.0, platform_half_width_m=30.0,
+                own_footprint=1)
+    terms = oracle.ray_terms(profile(), [], ends, [0.0] * 3, ALPHA)
+    assert abs(terms["slant_m"] - math.hypot(100.0, 50.0)) < 1e-9
+    # A deck below the terrain cuts the platform down to it.
+    low = oracle.vertical_path(profile(), [], 4.0, platform_half_width_m=30.0, own_footprint=1, source_ground_m=99.0)
+    np.testing.assert_allclose(low["z"], [99.0, 99.0, 105.0, 101.0, 103.0, 100.0], atol=1e-9)
+
+
 def test_own_footprint_skipped_and_tops_on_interpolated_terrain():
     """ray_tests.rs:107-134."""
     crossings = [building(0.05, 6.0, 1), building(0.1, 5.0, 3), building(0.12, 5.0, 3), barrier(0.6, 3.0, 2),
