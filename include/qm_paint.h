@@ -48,12 +48,14 @@ void qm_square_free(qm_square* square);
 /* Gpu::evaluate_pairs (paint-gpu/src/device.rs:440-765): for every pair (point, source) the
  * received energy per period mixed by the weather, then calm and downwind alone
  * (`sums`, 9 floats a pair: mixed[3], calm[3], favourable[3]; paint/src/exact.rs:129-133), its
- * failure bits, and every point's bands per layer, period and band (`point_bands`, optional,
- * n_points*6*3*8 floats, summed with the optional per-pair `pair_weight`; exact.rs:208-216).
- * Host arrays in and out. */
+ * failure bits, every point's bands per layer, period and band (`point_bands`, optional,
+ * n_points*6*3*8 floats, summed with the optional per-pair `pair_weight`; exact.rs:208-216), and
+ * each pair's own mixed bands (`pair_bands`, optional, n_pairs*3*8 floats, unweighted, zero for a
+ * failed pair: the `flat` output hum_pairs.rs reads for Batch::pair_spectra). Host arrays. */
 int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
                       const uint32_t* pair_point, const uint32_t* pair_source, const float* pair_weight,
-                      size_t n_pairs, float* sums, uint32_t* failed, float* point_bands, char* err, size_t errlen);
+                      size_t n_pairs, float* sums, uint32_t* failed, float* point_bands, float* pair_bands, char* err,
+                      size_t errlen);
 
 /* The painted grid: receivers on a row-major grid (`points`, nx*ny, x fastest), every source
  * within `cutoff_m` (horizontal) of each, in one persistent work-stealing launch. Out per
@@ -168,11 +170,12 @@ int qm_flight_sums_device(qm_gpu* gpu, const qm_flight_boxes* boxes, const qm_fl
  * count / scan / list of rays, and ray_transfers; popup/src/evaluate.rs:171-267 source_rays): per
  * pair (point, line piece) the piece's quadrature nodes seen from the point (physics/src/line.rs,
  * the skyline mask of wide buckets), one exact ray per node, and the pair's 9 sums, failure bits,
- * rays (optional, per pair) and point bands (optional, as qm_evaluate_pairs). Host arrays. */
+ * rays (optional, per pair), point bands and pair bands (optional, as qm_evaluate_pairs). Host
+ * arrays. */
 int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
                            const qm_line_piece* pieces, size_t n_pieces, const uint32_t* pair_point,
                            const uint32_t* pair_piece, size_t n_pairs, float* sums, uint32_t* failed, uint32_t* rays,
-                           float* point_bands, char* err, size_t errlen);
+                           float* point_bands, float* pair_bands, char* err, size_t errlen);
 
 /* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
  * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings

@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void line_bands_kernel(const uint32_t* pair_po
 extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
                                       const qm_line_piece* pieces, size_t n_pieces, const uint32_t* pair_point,
                                       const uint32_t* pair_piece, size_t n_pairs, float* sums, uint32_t* failed,
-                                      uint32_t* rays, float* point_bands, char* err, size_t errlen) {
+                                      uint32_t* rays, float* point_bands, float* pair_bands, char* err,
+                                      size_t errlen) {
     if (!gpu || !square || (n_pairs && (!points || !pieces || !pair_point || !pair_piece || !sums || !failed)))
         return fail(err, errlen, "null argument");
     if (n_pairs > 0xffffffffu || n_points > 0xffffffffu) return fail(err, errlen, "too many pairs or points");
@@ -209,19 +210,15 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     L_TRY(b.alloc(&d_nodes, n_pairs * LINE_NODE_CAP));
     L_TRY(b.alloc(&d_div, n_pairs));
     L_TRY(b.alloc(&d_sums, n_pairs * QM_PAIR_SUMS));
-    if (point_bands) {
-        L_TRY(b.alloc(&d_bands, n_points * QM_LAYERS * PERIODS * BANDS));
-        L_TRY(b.alloc(&d_pair_bands, n_pairs * PERIODS * BANDS));
-    }
+    if (point_bands) L_TRY(b.alloc(&d_bands, n_points * QM_LAYERS * PERIODS * BANDS));
+    if (point_bands || pair_bands) L_TRY(b.alloc(&d_pair_bands, n_pairs * PERIODS * BANDS));
     L_TRY(hipMemcpy(d_points, points, sizeof(qm_point) * n_points, hipMemcpyHostToDevice));
     L_TRY(hipMemcpy(d_pieces, pieces, sizeof(qm_line_piece) * n_pieces, hipMemcpyHostToDevice));
     L_TRY(hipMemcpy(d_pp, pair_point, 4 * n_pairs, hipMemcpyHostToDevice));
     L_TRY(hipMemcpy(d_pl, pair_piece, 4 * n_pairs, hipMemcpyHostToDevice));
     L_TRY(hipMemset(d_sums, 0, sizeof(float) * n_pairs * QM_PAIR_SUMS));
-    if (d_bands) {
-        L_TRY(hipMemset(d_bands, 0, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS));
-        L_TRY(hipMemset(d_pair_bands, 0, sizeof(float) * n_pairs * PERIODS * BANDS));
-    }
+    if (d_bands) L_TRY(hipMemset(d_bands, 0, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS));
+    if (d_pair_bands) L_TRY(hipMemset(d_pair_bands, 0, sizeof(float) * n_pairs * PERIODS * BANDS));
     hipLaunchKernelGGL(line_receivers_kernel, dim3((unsigned)((n_points + 255) / 256)), dim3(256), 0, 0, square->scene,
                        d_points, (uint32_t)n_points, d_receivers);
     hipLaunchKernelGGL(line_nodes_kernel, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, 0, square->scene,
@@ -269,6 +266,12 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     if (point_bands) {
         L_TRY(hipMemcpy(point_bands, d_bands, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS,
                         hipMemcpyDeviceToHost));
+    }
+    if (pair_bands) {
+        L_TRY(hipMemcpy(pair_bands, d_pair_bands, sizeof(float) * n_pairs * PERIODS * BANDS, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n_pairs; ++i)
+            if (failed[i])
+                for (int j = 0; j < PERIODS * BANDS; ++j) pair_bands[i * PERIODS * BANDS + j] = 0.0f;
     }
     return 0;
 }

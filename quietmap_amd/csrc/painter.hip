@@ -64,7 +64,8 @@ template <bool FINE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
-                                                    uint32_t n, float* sums, uint32_t* failed, float* point_bands) {
+                                                    uint32_t n, float* sums, uint32_t* failed, float* point_bands,
+                                                    float* pair_bands) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathScratch w;
@@ -82,6 +83,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WA
         out[q] = r.failed ? 0.0f : mixed;
         out[3 + q] = r.failed ? 0.0f : r.states[0][q];
         out[6 + q] = r.failed ? 0.0f : r.states[1][q];
+    }
+    if (pair_bands) {
+        float* dst = pair_bands + (size_t)i * PERIODS * BANDS;
+        for (int q = 0; q < PERIODS; ++q)
+            for (int b = 0; b < BANDS; ++b) dst[q * BANDS + b] = r.failed ? 0.0f : r.bands[q][b];
     }
     if (point_bands && !r.failed) {
         const float weight = pair_weight ? pair_weight[i] : 1.0f;
@@ -460,7 +466,8 @@ static int prepare_receivers(const qm_square* square, const qm_point* d_points, 
 
 int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* points, size_t n_points,
                       const uint32_t* pair_point, const uint32_t* pair_source, const float* pair_weight, size_t n_pairs,
-                      float* sums, uint32_t* failed, float* point_bands, char* err, size_t errlen) {
+                      float* sums, uint32_t* failed, float* point_bands, float* pair_bands, char* err,
+                      size_t errlen) {
     if (!gpu || !square || (n_pairs && (!pair_point || !pair_source || !sums || !failed)))
         return fail(err, errlen, "null argument");
     if (n_pairs > 0xffffffffu || n_points > 0xffffffffu) return fail(err, errlen, "too many pairs or points");
@@ -481,7 +488,7 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{stream};
-    DeviceBuffer d_points, d_receivers, d_pp, d_ps, d_pw, d_sums, d_failed, d_bands;
+    DeviceBuffer d_points, d_receivers, d_pp, d_ps, d_pw, d_sums, d_failed, d_bands, d_pair_bands;
     if (device_alloc(d_points, sizeof(qm_point) * n_points, err, errlen) ||
         device_alloc(d_receivers, sizeof(qm_receiver) * n_points, err, errlen) ||
         device_alloc(d_pp, sizeof(uint32_t) * n_pairs, err, errlen) ||
@@ -492,6 +499,7 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
     if (pair_weight && device_alloc(d_pw, sizeof(float) * n_pairs, err, errlen)) return 1;
     const size_t band_floats = n_points * QM_LAYERS * PERIODS * BANDS;
     if (point_bands && device_alloc(d_bands, sizeof(float) * band_floats, err, errlen)) return 1;
+    if (pair_bands && device_alloc(d_pair_bands, sizeof(float) * n_pairs * PERIODS * BANDS, err, errlen)) return 1;
     QM_TRY(hipMemcpyAsync(d_points.p, points, sizeof(qm_point) * n_points, hipMemcpyHostToDevice, stream));
     QM_TRY(hipMemcpyAsync(d_pp.p, pair_point, sizeof(uint32_t) * n_pairs, hipMemcpyHostToDevice, stream));
     QM_TRY(hipMemcpyAsync(d_ps.p, pair_source, sizeof(uint32_t) * n_pairs, hipMemcpyHostToDevice, stream));
@@ -506,17 +514,22 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
         hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
                            (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
                            (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr));
+                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
+                           (float*)(pair_bands ? d_pair_bands.p : nullptr));
     else
         hipLaunchKernelGGL(pairs_kernel<false>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
                            (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
                            (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr));
+                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
+                           (float*)(pair_bands ? d_pair_bands.p : nullptr));
     QM_TRY(hipGetLastError());
     QM_TRY(hipMemcpyAsync(sums, d_sums.p, sizeof(float) * n_pairs * QM_PAIR_SUMS, hipMemcpyDeviceToHost, stream));
     QM_TRY(hipMemcpyAsync(failed, d_failed.p, sizeof(uint32_t) * n_pairs, hipMemcpyDeviceToHost, stream));
     if (point_bands)
         QM_TRY(hipMemcpyAsync(point_bands, d_bands.p, sizeof(float) * band_floats, hipMemcpyDeviceToHost, stream));
+    if (pair_bands)
+        QM_TRY(hipMemcpyAsync(pair_bands, d_pair_bands.p, sizeof(float) * n_pairs * PERIODS * BANDS,
+                              hipMemcpyDeviceToHost, stream));
     QM_TRY(hipStreamSynchronize(stream));
     return 0;
 }

@@ -83,7 +83,7 @@ def library():
         lib.qm_square_bytes.argtypes = [vp]
         lib.qm_square_bytes.restype = sz
         lib.qm_square_free.argtypes = [vp]
-        lib.qm_evaluate_pairs.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, cp, sz]
+        lib.qm_evaluate_pairs.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_paint_grid.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, cp, sz]
         lib.qm_paint_grid_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
         lib.qm_receivers_device.argtypes = [vp, vp, vp, sz, vp, vp, cp, sz]
@@ -91,7 +91,7 @@ def library():
         lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
         lib.qm_loudness.argtypes = [vp, vp, sz, vp, cp, sz]
         lib.qm_loudness_device.argtypes = [vp, vp, sz, sz, sz, vp, vp, cp, sz]
-        lib.qm_evaluate_line_pairs.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, sz, vp, vp, vp, vp, cp, sz]
+        lib.qm_evaluate_line_pairs.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, sz, vp, vp, vp, vp, vp, cp, sz]
         lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, vp, vp,
                                          cp, sz]
         lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
@@ -204,7 +204,10 @@ class Gpu:
         _check(rc, err)
         return Square(self, handle, scene, sources)
 
-    def evaluate_pairs(self, square: Square, points, pair_point, pair_source, pair_weight=None, point_bands=True):
+    def evaluate_pairs(self, square: Square, points, pair_point, pair_source, pair_weight=None, point_bands=True,
+                       pair_bands=False):
+        """Pair sums [n, 9], failure bits, point bands [points, 6, 3, 8] and, with `pair_bands`,
+        each pair's own mixed bands [n, 3, 8] (Batch::pair_spectra's spectra)."""
         points = np.ascontiguousarray(points, abi.POINT_DTYPE)
         pp = np.ascontiguousarray(pair_point, np.uint32)
         ps = np.ascontiguousarray(pair_source, np.uint32)
@@ -213,11 +216,12 @@ class Gpu:
         sums = np.zeros((n, abi.PAIR_SUMS), np.float32)
         failed = np.zeros(n, np.uint32)
         bands = np.zeros((points.size, abi.LAYERS, abi.PERIODS, abi.BANDS), np.float32) if point_bands else None
+        spectra = np.zeros((n, abi.PERIODS, abi.BANDS), np.float32) if pair_bands else None
         err = C.create_string_buffer(512)
         rc = library().qm_evaluate_pairs(self.handle, square.handle, _ptr(points), points.size, _ptr(pp), _ptr(ps),
-                                         _ptr(pw), n, _ptr(sums), _ptr(failed), _ptr(bands), err, 512)
+                                         _ptr(pw), n, _ptr(sums), _ptr(failed), _ptr(bands), _ptr(spectra), err, 512)
         _check(rc, err)
-        return dict(sums=sums, failed=failed, point_bands=bands)
+        return dict(sums=sums, failed=failed, point_bands=bands, pair_bands=spectra)
 
     def paint_grid(self, square: Square, points, nx: int, ny: int, cutoff_m: float):
         points = np.ascontiguousarray(points, abi.POINT_DTYPE)
@@ -366,8 +370,10 @@ class Gpu:
                                              n_segments, d_sums, stream or None, err, 512)
         _check(rc, err)
 
-    def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False):
-        """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands."""
+    def evaluate_line_pairs(self, square: Square, points, pieces, pair_point, pair_piece, point_bands=False,
+                            pair_bands=False):
+        """Line pairs (point, piece): sums [n, 9], failure bits, rays per pair, point bands and pair
+        bands."""
         pts = np.ascontiguousarray(points, abi.POINT_DTYPE)
         pcs = np.ascontiguousarray(pieces, abi.LINE_PIECE_DTYPE)
         pp = np.ascontiguousarray(pair_point, np.uint32)
@@ -377,12 +383,13 @@ class Gpu:
         failed = np.zeros(n, np.uint32)
         rays = np.zeros(n, np.uint32)
         bands = np.zeros((pts.size, abi.LAYERS, 3, 8), np.float32) if point_bands else None
+        spectra = np.zeros((n, 3, 8), np.float32) if pair_bands else None
         err = C.create_string_buffer(512)
         rc = library().qm_evaluate_line_pairs(self.handle, square.handle, _ptr(pts), pts.size, _ptr(pcs), pcs.size,
                                               _ptr(pp), _ptr(pl), n, _ptr(sums), _ptr(failed), _ptr(rays),
-                                              _ptr(bands), err, 512)
+                                              _ptr(bands), _ptr(spectra), err, 512)
         _check(rc, err)
-        return dict(sums=sums, failed=failed, rays=rays, point_bands=bands)
+        return dict(sums=sums, failed=failed, rays=rays, point_bands=bands, pair_bands=spectra)
 
     def crossings(self, square: Square, segments, capacity: int = 48):
         seg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)
