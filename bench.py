@@ -42,8 +42,8 @@ def main():
     ap.add_argument("--loudness-cells", type=int, default=16 * 1024 * 1024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (tiles shard with no exchange; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the painter has no multi-GPU path (tiles of receivers shard "
+              "across GPUs with no exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
