@@ -39,6 +39,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loudness", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-flights", action="store_true")
     ap.add_argument("--loudness-cells", type=int, default=16 * 1024 * 1024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -140,6 +141,53 @@ def main():
         print(f"[bench] surrogate {sur['value']:.3e} paths/s ({sur['speedup_vs_exact']:.2f}x exact), Lden vs exact "
               f"mean {sur['lden_vs_exact_db']['mean']:.3f} max {sur['lden_vs_exact_db']['max']:.3f} dB",
               file=sys.stderr, flush=True)
+
+    # The aircraft-box leg: 20,000 boxes over 20 km heard at a 64 x 64 grid of points (every point
+    # hearing all 16 tiles), inputs resident on the device, one flight_sums launch a step.
+    fl = None
+    if not args.no_flights:
+        from quietmap_amd import flights
+
+        boxes_f = flights.synthetic_boxes(seed=31, count=20000, extent_m=20000.0)
+        side = 64
+        jj, ii = np.mgrid[0:side, 0:side]
+        xy_f = np.stack([(ii.reshape(-1) + 0.5) * 20000.0 / side, (jj.reshape(-1) + 0.5) * 20000.0 / side], 1)
+        boxes_f, points_f, segments_f, n_sums_f = flights.flight_requests(boxes_f, xy_f, extent_m=20000.0)
+        handle_f = gpu.flight_boxes(boxes_f)
+        d_fp, d_fs = hip.DeviceBuffer.from_array(points_f), hip.DeviceBuffer.from_array(segments_f)
+        d_sums_f = hip.DeviceBuffer(n_sums_f * 291 * 8)
+        reads = int(segments_f["count"].astype(np.int64).sum())
+        ft = []
+        for k in range(4):
+            hip.check(hip.runtime().hipMemset(d_sums_f.ptr, 0, n_sums_f * 291 * 8))
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            gpu.flight_sums_device(handle_f, d_fp.ptr, 0, d_fs.ptr, segments_f.size, d_sums_f.ptr, stream.handle)
+            e1.record(stream)
+            e1.synchronize()
+            if k:
+                ft.append(e1.seconds_since(e0))
+        f_time = sum(ft) / len(ft)
+        sums_f = d_sums_f.to_array(np.float64, (n_sums_f, 291))
+        fl = {"value": reads / f_time, "unit": "box reads/s", "kernel_ms": f_time * 1e3, "boxes": int(boxes_f.size),
+              "points": int(points_f.size), "segments": int(segments_f.size), "kernel": "flight_sums_kernel"}
+        if not args.no_cpu:
+            from oracle import oracle
+
+            pick = np.arange(0, points_f.size, 64)  # 64 points' segments
+            keep = np.isin(segments_f["point"], pick)
+            t1 = time.time()
+            want_f = oracle.flight_sums(boxes_f, points_f, None, segments_f[keep], n_sums_f)
+            cpu_f = int(segments_f["count"][keep].astype(np.int64).sum()) / (time.time() - t1)
+            fl["cpu_baseline"] = {"value": cpu_f, "unit": "box reads/s", "cores": 1, "kind": "port",
+                                  "sample": f"the {int(keep.sum())} segments of {pick.size} points, FP64 C++ "
+                                            "restatement, one thread"}
+            e_g, e_w = sums_f[pick, :3], want_f[pick, :3]
+            ok = e_w > 0
+            fl["max_abs_db_vs_oracle"] = float(np.abs(10 * np.log10(e_g[ok] / e_w[ok])).max())
+            fl["classes_equal"] = bool(np.array_equal(sums_f[pick, 3 + 144:] > 0, want_f[pick, 3 + 144:] > 0))
+        handle_f.close()
+        print(f"[bench] flights {fl['value']:.3e} box reads/s", file=sys.stderr, flush=True)
 
     # The loudness leg: ISO 532-1 on band-major planes of 28 third-octave levels per cell.
     loud = None
@@ -267,6 +315,7 @@ def main():
         "parity_vs_fp64_oracle": parity,
         "loudness": loud,
         "surrogate": sur,
+        "flights": fl,
         "gpu": gpu.name,
     }
     print(json.dumps(line), flush=True)

@@ -22,6 +22,7 @@ def runtime():
         hip.hipMalloc.argtypes = [C.POINTER(vp), sz]
         hip.hipFree.argtypes = [vp]
         hip.hipMemcpy.argtypes = [vp, vp, sz, C.c_int]
+        hip.hipMemset.argtypes = [vp, C.c_int, sz]
         hip.hipDeviceSynchronize.argtypes = []
         hip.hipEventCreate.argtypes = [C.POINTER(vp)]
         hip.hipEventRecord.argtypes = [vp, vp]
