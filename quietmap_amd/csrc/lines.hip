@@ -76,8 +76,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_rays) return;
     PathScratch w;
-    StateStream states[2];
-    w.state = states;
     const uint32_t i = ray_pair[k];
     const LineNode node = nodes[(size_t)i * LINE_NODE_CAP + (k - ray_first[i])];
     const qm_line_piece& piece = pieces[pair_piece[i]];

@@ -69,8 +69,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WA
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathScratch w;
-    StateStream states[2];
-    w.state = states;
     PathResult r;
     const uint32_t p = pair_point[i];
     const qm_source src = sources[pair_source[i]];
@@ -114,11 +112,7 @@ __device__ uint32_t next_tile(const GridArgs& g, int xcd) {
 template <bool FINE>
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
     __shared__ uint32_t s_tile;
-    // The two states' scalars of every work-item in LDS: touched at every candidate, they stay
-    // out of the scratch traffic of the per-path buffers.
-    __shared__ StateStream s_states[GROUP][2];
     PathScratch w;
-    w.state = s_states[threadIdx.x];
     const int lane = threadIdx.x;
     const int wave = lane >> 6, l = lane & 63;
     const int lx = (wave & 1) * 8 + (l & 7), ly = (wave >> 1) * 8 + (l >> 3);

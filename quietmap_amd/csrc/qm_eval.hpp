@@ -13,7 +13,6 @@ struct PathScratch {
     Roof roofs[ROOF_CAP];
     CandLog log;
     Hull hull[2];
-    StateStream* state;  // the two states' scalars (the caller's: registers, scratch or LDS)
 };
 
 // The crest thinning of physics/src/ray.rs:360-385: the open group's best sample.
@@ -45,8 +44,8 @@ QM_DEV float favourable_at(const float* row, float azimuth) {
 QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
     if (!(c.x > 0.0f && c.x < length)) return;
     c.id = -1;
-    state_candidate(w.state, w.hull, 0, w.log, S, R, c, failed);
-    state_candidate(w.state, w.hull, 1, w.log, S, R, c, failed);
+    state_candidate(lane_states(), w.hull, 0, w.log, S, R, c, failed);
+    state_candidate(lane_states(), w.hull, 1, w.log, S, R, c, failed);
 }
 
 // The x of the first top at or after index i not dropped (+inf when none).
@@ -173,7 +172,7 @@ QM_DEV void foliage_sample(Foliage& f, const PathScratch& w, float x, float grou
         return;
     }
     for (int s = 0; s < 2; ++s) {
-        const float za = line_altitude(w.state[s].line, f.prev_x), zb = line_altitude(w.state[s].line, x);
+        const float za = line_altitude(lane_states()[s].line, f.prev_x), zb = line_altitude(lane_states()[s].line, x);
         const float above_a = za - f.prev_ground, above_b = zb - ground;
         const float wa = (above_a > 0.0f && above_a <= f.prev_canopy) ? f.prev_cover : 0.0f;
         const float wb = (above_b > 0.0f && above_b <= canopy) ? cover : 0.0f;
@@ -477,18 +476,18 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const P2f R{H, receiver_alt};
     {
         const float direct = sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt));
-        w.state[0].radius = 0.0f;
-        w.state[1].radius = fmaxf(1000.0f, 8.0f * direct);
+        lane_states()[0].radius = 0.0f;
+        lane_states()[1].radius = fmaxf(1000.0f, 8.0f * direct);
         for (int s = 0; s < 2; ++s) {
-            w.state[s].line = state_line(w.state[s].radius, S, R);
-            w.state[s].arc_sr = arc_excess_f(w.state[s].radius, w.state[s].line.chord);
+            lane_states()[s].line = state_line(lane_states()[s].radius, S, R);
+            lane_states()[s].arc_sr = arc_excess_f(lane_states()[s].radius, lane_states()[s].line.chord);
         }
         for (int s = 0; s < 2; ++s) {
-            w.state[s].blocking = false;
-            w.state[s].has_pending = false;
-            w.state[s].have_best = false;
-            w.state[s].hull_n = 0;
-            w.state[s].best_delta = 0.0f;
+            lane_states()[s].blocking = false;
+            lane_states()[s].has_pending = false;
+            lane_states()[s].have_best = false;
+            lane_states()[s].hull_n = 0;
+            lane_states()[s].best_delta = 0.0f;
         }
     }
     w.log.n = 0;
@@ -630,7 +629,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float near_source = near_source_factor(direct, path_ground, source_g);
     float att[2][BANDS];
     for (int s = 0; s < 2; ++s)
-        state_boundary(B, w.state[s], w.hull[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
+        state_boundary(B, lane_states()[s], w.hull[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
                        near_source, source_g, s0.g, failed, att[s]);
     for (int b = 0; b < BANDS; ++b) {
         out.att[0][b] = att[0][b];

@@ -771,6 +771,13 @@ struct StateStream {
     CandRef best;
 };
 
+// The two states of every work-item of a group (groups of at most 256 work-items), in LDS: the
+// scalars touched at every candidate, read and written with LDS instructions wherever the walk
+// is inlined (a pointer to them carried in the per-path buffers made them generic accesses that
+// waited on the scratch traffic).
+static __shared__ StateStream qm_lane_states[256][2];
+QM_DEV StateStream* lane_states() { return qm_lane_states[threadIdx.x]; }
+
 // A state's hull: the diffraction points of its rubber band so far, source excluded.
 struct Hull {
     CandRef e[HULL_CAP];
@@ -893,6 +900,10 @@ QM_DEV Vertex extend(const Vertex& a, float x, float z, float g, double z0) {
 
 // The profile at x on the segment from `a` toward (bx, bz, bg), a.x <= x < bx (the upper side of
 // a step: the last vertex at or before x, interpolated toward the next), with the prefix to x.
+// The prefix takes the point on the segment in FP64: a receiver-side plane over the few
+// centimetres from a wall to the receiver is fitted from the difference of the end's prefix and
+// this one, both spanning the whole segment, and an FP32-rounded point there (1e-5 m at 200 m)
+// turned that difference, and the fitted slope, into noise.
 QM_DEV void segment_at(const Vertex& a, float bx, float bz, float bg, float x, double z0, Cand& c) {
     if (a.x == x) {
         c.z_at = a.z;
@@ -903,14 +914,16 @@ QM_DEV void segment_at(const Vertex& a, float bx, float bz, float bg, float x, d
         return;
     }
     const float f = (x - a.x) / (bx - a.x);
-    const float z = a.z + f * (bz - a.z);
-    const float g = a.g + f * (bg - a.g);
-    const Vertex v = extend(a, x, z, g, z0);
-    c.z_at = z;
-    c.g_at = g;
-    c.pz = v.pz;
-    c.pxz = v.pxz;
-    c.pg = v.pg;
+    c.z_at = a.z + f * (bz - a.z);
+    c.g_at = a.g + f * (bg - a.g);
+    const double u0 = a.x, u1 = x;
+    const double fd = (u1 - u0) / ((double)bx - u0);
+    const double w0 = (double)a.z - z0, w1 = (double)a.z + fd * ((double)bz - (double)a.z) - z0;
+    const double gd = (double)a.g + fd * ((double)bg - (double)a.g);
+    const double width = u1 - u0;
+    c.pz = a.pz + 0.5 * (w0 + w1) * width;
+    c.pxz = a.pxz + width * (u0 * (2.0 * w0 + w1) + u1 * (w0 + 2.0 * w1)) * (1.0 / 6.0);
+    c.pg = a.pg + 0.5 * ((double)a.g + gd) * width;
 }
 
 // The profile at a walk vertex's x just after the walk has handled it: the vertex itself when it

@@ -58,8 +58,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathScratch w;
-    StateStream states[2];
-    w.state = states;
     const qm_receiver& rcv = receivers[pair_receiver[i]];
     const qm_source src = sources[pair_source[i]];
     float f[FEATURES];
@@ -196,8 +194,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void e
     const uint32_t* exact_list, const uint32_t* exact_count, float* exact_energy, uint8_t* exact_failed) {
     const uint32_t count = *exact_count;
     PathScratch w;
-    StateStream states[2];
-    w.state = states;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
         const uint32_t pair = exact_list[k];
         const uint32_t item = pair / 256, lane = pair % 256;

@@ -52,9 +52,7 @@ struct TilesArgs {
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void tiles_kernel(Scene sc, BandTable B,
                                                                                                 TilesArgs g) {
     __shared__ uint32_t s_item;
-    __shared__ StateStream s_states[GROUP][2];
     PathScratch w;
-    w.state = s_states[threadIdx.x];
     const int lane = threadIdx.x;
     const int wave = lane >> 6, l = lane & 63;
     const int lx = (wave & 1) * 8 + (l & 7), ly = (wave >> 1) * 8 + (l >> 3);
