@@ -55,9 +55,11 @@ QM_DEV float top_x_from(const PathScratch& w, int& i, int n) {
 }
 
 // Emits every resolved top before x (the candidate merge of physics/src/cnossos/mod.rs:160-184:
-// a top first only when strictly nearer).
-__device__ __noinline__ void tops_before_slow(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x,
-                                              uint32_t& failed) {
+// a top first only when strictly nearer). Inlined: out of line, its accesses to the per-path
+// buffers went through generic pointers (flat instructions waiting on all memory traffic), which
+// cost the bench step 30 %.
+__device__ void tops_before_slow(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x,
+                                 uint32_t& failed) {
     while (tp.emit_x < x && tp.emitted < tp.resolved) {
         const Crossing& c = w.cross[tp.emitted];
         Cand k;
@@ -79,8 +81,7 @@ QM_DEV void tops_before(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float len
 
 // Reads the profile for every top before x on the segment from the profile's head to the vertex
 // (x, z, g) about to be emitted.
-__device__ __noinline__ void resolve_tops_slow(PathScratch& w, Tops& tp, int n, const Profile& P, float x, float z,
-                                               float g) {
+__device__ void resolve_tops_slow(PathScratch& w, Tops& tp, int n, const Profile& P, float x, float z, float g) {
     while (tp.resolve_x < x) {
         Crossing& c = w.cross[tp.resolved];
         Cand k;
