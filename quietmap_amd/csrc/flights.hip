@@ -40,11 +40,6 @@ constexpr double SHARE_NEAR_M = 1000.0 * METRES_PER_FOOT;  // box_sums.rs INSTAL
 constexpr double SHARE_FAR_M = TAIL_ANCHOR_M;
 
 struct FlightArgs {
-    const qm_flight_box* boxes;
-    const qm_flight_point* points;
-    const float* horizons;
-    const qm_flight_segment* segments;
-    double* sums;
     float log_npd[ND];
     double log_npd_d[ND];
     double seconds[3];
@@ -270,21 +265,28 @@ __device__ int pass_class_exact(double energy, double passing) {
     return min(c, QM_PASS_CLASSES - 1);
 }
 
-__global__ __launch_bounds__(GROUP) void flight_sums_kernel(FlightArgs g) {
+// The arrays come as kernel arguments of their own (global pointers), the tables in `g`: pointers
+// read out of an argument struct that is also passed by reference were generic, and every box
+// field a flat load.
+__global__ __launch_bounds__(GROUP) void flight_sums_kernel(const qm_flight_box* __restrict__ boxes,
+                                                            const qm_flight_point* __restrict__ points,
+                                                            const float* __restrict__ horizons,
+                                                            const qm_flight_segment* __restrict__ segments,
+                                                            double* __restrict__ sums, FlightArgs g) {
     __shared__ double classes[2 * CLASS_SLOTS];  // energies [period][class], then passing shares
     __shared__ double partial[GROUP / 64][3];
     const int tid = threadIdx.x;
     for (int i = tid; i < 2 * CLASS_SLOTS; i += GROUP) classes[i] = 0.0;
     __syncthreads();
-    const qm_flight_segment sg = g.segments[blockIdx.x];
-    const qm_flight_point& pt = g.points[sg.point];
+    const qm_flight_segment sg = segments[blockIdx.x];
+    const qm_flight_point& pt = points[sg.point];
     const double px = pt.x, py = pt.y, palt = pt.altitude_m;
-    const float* horizon = pt.horizons == QM_NO_HORIZONS ? nullptr : g.horizons + pt.horizons;
+    const float* horizon = pt.horizons == QM_NO_HORIZONS ? nullptr : horizons + pt.horizons;
     const double reach = fmin(sg.within_m, QM_AIRCRAFT_REACH_M);
     const double impedance_point = (double)pt.impedance_gain;
     double energy[3] = {0.0, 0.0, 0.0};
     for (uint32_t k = tid; k < sg.count; k += GROUP) {
-        const qm_flight_box& b = g.boxes[sg.first + k];
+        const qm_flight_box& b = boxes[sg.first + k];
         const double east = b.east_m - px, north = b.north_m - py;
         const double distance = hypot(east, north);
         if (distance > reach) continue;
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(GROUP) void flight_sums_kernel(FlightArgs g) {
         if ((tid & 63) == 0) partial[tid >> 6][p] = v;
     }
     __syncthreads();
-    double* at = g.sums + (size_t)sg.sum * QM_FLIGHT_SUMS;
+    double* at = sums + (size_t)sg.sum * QM_FLIGHT_SUMS;
     if (tid < 3) {
         double total = 0.0;
         for (int w = 0; w < GROUP / 64; ++w) total += partial[w][tid];
@@ -424,13 +426,9 @@ extern "C" int qm_flight_sums_device(qm_gpu* gpu, const qm_flight_boxes* boxes, 
     if (n_segments == 0) return 0;
     hipError_t e = hipSetDevice(gpu->ordinal);
     if (e == hipSuccess) {
-        FlightArgs g = args_of();
-        g.boxes = boxes->d_boxes;
-        g.points = d_points;
-        g.horizons = d_horizons;
-        g.segments = d_segments;
-        g.sums = d_sums;
-        hipLaunchKernelGGL(flight_sums_kernel, dim3((unsigned)n_segments), dim3(GROUP), 0, (hipStream_t)stream, g);
+        const FlightArgs g = args_of();
+        hipLaunchKernelGGL(flight_sums_kernel, dim3((unsigned)n_segments), dim3(GROUP), 0, (hipStream_t)stream,
+                           (const qm_flight_box*)boxes->d_boxes, d_points, d_horizons, d_segments, d_sums, g);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return ffail(err, errlen, "flight sums: %s", hipGetErrorString(e));
