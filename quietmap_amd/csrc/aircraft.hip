@@ -182,11 +182,15 @@ __device__ void row_read_d(const DevRow& r, int interval, double fraction, doubl
 
 // The SEL (dB) of segment (a, b) in FP64, in the oracle's operation order (segment.rs:318-350 over
 // the closest points, SegmentEmission::read_npd and corrections.rs), unscreened.
-__device__ __noinline__ double segment_sel_d(const AircraftArgs& g, const DevTrack& tr, const DevRow* rows, const float* a,
-                                const float* b, double rx, double ry, const float* horizon) {
+// It takes the receiver altitude and the NPD distances' logarithms by value and the track by a
+// pointer to the caller's copy: a reference into the kernel's argument block would make every
+// pointer the kernel reads from that block a generic one.
+__device__ __noinline__ double segment_sel_d(double receiver_altitude_m, const double* log_npd_d, const DevTrack& tr,
+                                             const DevRow* rows, const float* a, const float* b, double rx, double ry,
+                                             const float* horizon) {
 #pragma clang fp contract(off)
-    const double s[3] = {(double)a[0] - rx, (double)a[1] - ry, (double)a[2] - g.receiver_altitude_m};
-    const double e[3] = {(double)b[0] - rx, (double)b[1] - ry, (double)b[2] - g.receiver_altitude_m};
+    const double s[3] = {(double)a[0] - rx, (double)a[1] - ry, (double)a[2] - receiver_altitude_m};
+    const double e[3] = {(double)b[0] - rx, (double)b[1] - ry, (double)b[2] - receiver_altitude_m};
     const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
     const double hs = d[0] * d[0] + d[1] * d[1];
     const double ls = hs + d[2] * d[2];
@@ -217,12 +221,11 @@ __device__ __noinline__ double segment_sel_d(const AircraftArgs& g, const DevTra
     const double log_d = log10(slant / METRES_PER_FOOT);
     int interval = 0;
     for (int k = ND - 2; k >= 1; --k)
-        if (log_d > g.tables.log_npd_d[k]) {
+        if (log_d > log_npd_d[k]) {
             interval = k;
             break;
         }
-    const double fraction =
-        (log_d - g.tables.log_npd_d[interval]) / (g.tables.log_npd_d[interval + 1] - g.tables.log_npd_d[interval]);
+    const double fraction = (log_d - log_npd_d[interval]) / (log_npd_d[interval + 1] - log_npd_d[interval]);
     double sel, scaled;
     row_read_d(rows[tr.row_index], interval, fraction, slant, sel, scaled);
     if (tr.weight_d != 0.0) {
@@ -471,8 +474,14 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
                         const bool near_edge =
                             edge >= 1.0f && edge <= (float)(QM_PASS_CLASSES - 1) &&
                             fabsf(level - ((float)QM_PASS_LOWEST_DB + (float)QM_PASS_CLASS_DB * edge)) < EPS_DB;
-                        const double exact =
-                            near_edge ? energy_d(segment_sel_d(g, tr, g.rows, pa, pa + 3, rx, ry, screen)) : 0.0;
+                        double exact = 0.0;
+                        if (near_edge) {
+                            double log_npd_d[ND];
+                            for (int n = 0; n < ND; ++n) log_npd_d[n] = g.tables.log_npd_d[n];
+                            const DevTrack track = tr;
+                            exact = energy_d(segment_sel_d(g.receiver_altitude_m, log_npd_d, track, g.rows, pa, pa + 3,
+                                                           rx, ry, screen));
+                        }
                         for (int p = 0; p < 3; ++p) {
                             const double seconds = g.seconds[p];
                             const double e_p = (double)tr.flights[p] * (double)sf.sel_energy / seconds;
