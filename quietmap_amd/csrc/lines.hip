@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void line_nodes_kernel(Scene sc, const qm_rece
 }
 
 template <bool FINE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void line_rays_kernel(
+__global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(2))) void line_rays_kernel(
     Scene sc, BandTable B, const qm_receiver* receivers, const qm_line_piece* pieces, const uint32_t* pair_point,
     const uint32_t* pair_piece, const uint32_t* ray_pair, const uint32_t* ray_first, uint32_t n_rays,
     const LineNode* nodes, const float* divergence, float* sums, uint32_t* failed, float* pair_bands) {
@@ -182,6 +182,8 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
         if (!std::isfinite(pieces[i].ax_m) || !std::isfinite(pieces[i].ay_m) || !std::isfinite(pieces[i].bx_m) ||
             !std::isfinite(pieces[i].by_m))
             return fail(err, errlen, "piece %zu: ends not finite", i);
+    for (size_t i = 0; i < n_pieces; ++i)
+        if (pieces[i].layer >= QM_LAYERS) return fail(err, errlen, "piece %zu: layer out of range", i);
     for (size_t i = 0; i < n_pairs; ++i) {
         if (pair_point[i] >= n_points) return fail(err, errlen, "pair %zu: point %u out of range", i, pair_point[i]);
         if (pair_piece[i] >= n_pieces) return fail(err, errlen, "pair %zu: piece %u out of range", i, pair_piece[i]);
@@ -239,11 +241,11 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     L_TRY(hipMemcpy(d_ray_first, first.data(), 4 * n_pairs, hipMemcpyHostToDevice));
     if (total) {
         if (square->scene.terrain.fine_detail)
-            hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0,
+            hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
                                square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
                                (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
         else
-            hipLaunchKernelGGL(line_rays_kernel<false>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0,
+            hipLaunchKernelGGL(line_rays_kernel<false>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
                                square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
                                (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
         L_TRY(hipGetLastError());

@@ -23,14 +23,14 @@
 #include "../../include/qm_paint.h"
 #include "qm_eval.hpp"
 #include "qm_internal.hpp"
+#include "qm_queue.hpp"
 
 using namespace qm;
 
 namespace {
 
-constexpr int XCDS = 8;
 constexpr int TILE = 16;
-constexpr int GROUP = 256;
+constexpr int GROUP = QM_PATH_GROUP;
 
 struct GridArgs {
     const qm_receiver* receivers;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 // FINE: the square holds fine terrain (its own instantiation, so the coarse kernel keeps its
 // registers).
 template <bool FINE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
+__global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
                                                     uint32_t n, float* sums, uint32_t* failed, float* point_bands,
@@ -97,20 +97,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WA
     }
 }
 
-// The next tile for a group on XCD `xcd`: its own stripe's queue first, then the others'.
-__device__ uint32_t next_tile(const GridArgs& g, int xcd) {
-    for (int k = 0; k < XCDS; ++k) {
-        const int q = (xcd + k) % XCDS;
-        const uint32_t begin = (uint32_t)(((uint64_t)g.n_tiles * q) / XCDS);
-        const uint32_t end = (uint32_t)(((uint64_t)g.n_tiles * (q + 1)) / XCDS);
-        if (end <= begin) continue;
-        if (__hip_atomic_load(&g.counters[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end - begin) continue;
-        const uint32_t t = atomicAdd(&g.counters[q], 1u);
-        if (t < end - begin) return begin + t;
-    }
-    return g.n_tiles;
-}
-
 template <bool FINE>
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
     __shared__ uint32_t s_tile;
@@ -120,7 +106,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
     const int lx = (wave & 1) * 8 + (l & 7), ly = (wave >> 1) * 8 + (l >> 3);
     const int xcd = blockIdx.x % XCDS;
     for (;;) {
-        if (lane == 0) s_tile = next_tile(g, xcd);
+        if (lane == 0) s_tile = next_tile(g.counters, g.n_tiles, xcd);
         __syncthreads();
         const uint32_t tile = s_tile;
         __syncthreads();
@@ -177,11 +163,6 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
             g.failures[index] = fails;
         }
     }
-}
-
-__global__ void zero_kernel(uint32_t* p, uint32_t n) {
-    const uint32_t i = threadIdx.x;
-    if (i < n) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void crossings_kernel(Scene sc, const double* segments, uint32_t n, uint32_t capacity,
@@ -273,6 +254,54 @@ int device_alloc(DeviceBuffer& b, size_t bytes, char* err, size_t errlen) {
 
 int check_device(qm_gpu* gpu, char* err, size_t errlen) {
     QM_TRY(hipSetDevice(gpu->ordinal));
+    return 0;
+}
+
+// Device memory freed on the stream at scope end (hipMallocAsync's pool).
+struct AsyncBuffer {
+    void* p = nullptr;
+    hipStream_t stream = nullptr;
+    ~AsyncBuffer() {
+        if (p) (void)hipFreeAsync(p, stream);
+    }
+};
+
+// The grid painter over prepared receivers (nx*ny, row-major), its XCDS queue heads at
+// `d_counters` cleared first by a kernel on the stream (no memset node where a caller captures).
+int launch_paint(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_receivers, uint32_t nx, uint32_t ny,
+                 double cutoff_m, double* d_period_energy, uint32_t* d_paths, uint32_t* d_failures,
+                 uint32_t* d_counters, hipStream_t stream, char* err, size_t errlen) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, d_counters, (uint32_t)XCDS);
+    QM_TRY(hipGetLastError());
+    GridArgs g;
+    g.receivers = d_receivers;
+    g.nx = nx;
+    g.ny = ny;
+    g.tiles_x = (nx + TILE - 1) / TILE;
+    g.tiles_y = (ny + TILE - 1) / TILE;
+    g.n_tiles = g.tiles_x * g.tiles_y;
+    g.cutoff = cutoff_m;
+    g.energy = d_period_energy;
+    g.paths = d_paths;
+    g.failures = d_failures;
+    g.counters = d_counters;
+    int cus = 0;
+    QM_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal));
+    int per_cu = 0;
+    const bool fine = square->scene.terrain.fine_detail != nullptr;
+    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, fine ? reinterpret_cast<const void*>(paint_kernel<true>) : reinterpret_cast<const void*>(paint_kernel<false>),
+        GROUP, 0));
+    unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
+    groups = std::min<unsigned>(groups, g.n_tiles);
+    groups = std::max<unsigned>(groups, 1);
+    if (fine)
+        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+                           square->grid, g);
+    else
+        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+                           square->grid, g);
+    QM_TRY(hipGetLastError());
     return 0;
 }
 
@@ -540,14 +569,13 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
     if (n == 0) return 0;
     if (check_device(gpu, err, errlen)) return 1;
     hipStream_t stream = (hipStream_t)stream_ptr;
-    qm_receiver* d_receivers = nullptr;
-    QM_TRY(hipMallocAsync((void**)&d_receivers, sizeof(qm_receiver) * n, stream));
-    if (prepare_receivers(square, d_points, n, d_receivers, stream, err, errlen)) return 1;
-    if (qm_paint_receivers_device(gpu, square, d_receivers, nx, ny, cutoff_m, d_period_energy, d_paths, d_failures,
-                                  stream, err, errlen))
-        return 1;
-    QM_TRY(hipFreeAsync(d_receivers, stream));
-    return 0;
+    AsyncBuffer receivers, counters;
+    receivers.stream = counters.stream = stream;
+    QM_TRY(hipMallocAsync(&receivers.p, sizeof(qm_receiver) * n, stream));
+    QM_TRY(hipMallocAsync(&counters.p, sizeof(uint32_t) * XCDS, stream));
+    if (prepare_receivers(square, d_points, n, (qm_receiver*)receivers.p, stream, err, errlen)) return 1;
+    return launch_paint(gpu, square, (const qm_receiver*)receivers.p, nx, ny, cutoff_m, d_period_energy, d_paths,
+                        d_failures, (uint32_t*)counters.p, stream, err, errlen);
 }
 
 int qm_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, size_t n,
@@ -567,42 +595,11 @@ int qm_paint_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_rec
     if ((size_t)nx * ny == 0) return 0;
     if (check_device(gpu, err, errlen)) return 1;
     hipStream_t stream = (hipStream_t)stream_ptr;
-    // The queue heads, zeroed on the stream by a kernel (no memset node under capture).
-    uint32_t* d_counters = nullptr;
-    QM_TRY(hipMallocAsync((void**)&d_counters, sizeof(uint32_t) * XCDS, stream));
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, d_counters, (uint32_t)XCDS);
-    QM_TRY(hipGetLastError());
-    GridArgs g;
-    g.receivers = d_receivers;
-    g.nx = nx;
-    g.ny = ny;
-    g.tiles_x = (nx + TILE - 1) / TILE;
-    g.tiles_y = (ny + TILE - 1) / TILE;
-    g.n_tiles = g.tiles_x * g.tiles_y;
-    g.cutoff = cutoff_m;
-    g.energy = d_period_energy;
-    g.paths = d_paths;
-    g.failures = d_failures;
-    g.counters = d_counters;
-    int cus = 0;
-    QM_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal));
-    int per_cu = 0;
-    const bool fine = square->scene.terrain.fine_detail != nullptr;
-    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, fine ? reinterpret_cast<const void*>(paint_kernel<true>) : reinterpret_cast<const void*>(paint_kernel<false>),
-        GROUP, 0));
-    unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
-    groups = std::min<unsigned>(groups, g.n_tiles);
-    groups = std::max<unsigned>(groups, 1);
-    if (fine)
-        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
-                           square->grid, g);
-    else
-        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
-                           square->grid, g);
-    QM_TRY(hipGetLastError());
-    QM_TRY(hipFreeAsync(d_counters, stream));
-    return 0;
+    AsyncBuffer counters;
+    counters.stream = stream;
+    QM_TRY(hipMallocAsync(&counters.p, sizeof(uint32_t) * XCDS, stream));
+    return launch_paint(gpu, square, d_receivers, nx, ny, cutoff_m, d_period_energy, d_paths, d_failures,
+                        (uint32_t*)counters.p, stream, err, errlen);
 }
 
 int qm_paint_grid(qm_gpu* gpu, const qm_square* square, const qm_point* points, uint32_t nx, uint32_t ny,

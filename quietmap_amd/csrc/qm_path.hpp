@@ -771,11 +771,15 @@ struct StateStream {
     CandRef best;
 };
 
-// The two states of every work-item of a group (groups of at most 256 work-items), in LDS: the
-// scalars touched at every candidate, read and written with LDS instructions wherever the walk
-// is inlined (a pointer to them carried in the per-path buffers made them generic accesses that
-// waited on the scratch traffic).
-static __shared__ StateStream qm_lane_states[256][2];
+// Work-items of a group of every kernel that inlines the path walk: 1-D groups of at most this
+// many (each such kernel's __launch_bounds__ names it, and its launches are dim3(n) x dim3(k <=
+// QM_PATH_GROUP)). The LDS below costs sizeof(StateStream) x 2 x QM_PATH_GROUP bytes a group.
+#define QM_PATH_GROUP 256
+
+// The two states of every work-item of a group, in LDS: the scalars touched at every candidate,
+// read and written with LDS instructions wherever the walk is inlined (a pointer to them carried
+// in the per-path buffers made them generic accesses that waited on the scratch traffic).
+static __shared__ StateStream qm_lane_states[QM_PATH_GROUP][2];
 QM_DEV StateStream* lane_states() { return qm_lane_states[threadIdx.x]; }
 
 // A state's hull: the diffraction points of its rubber band so far, source excluded.

@@ -28,7 +28,7 @@ using namespace qm;
 
 namespace {
 
-constexpr int GROUP = 256;
+constexpr int GROUP = QM_PATH_GROUP;
 constexpr int BLOCK = 16;  // receiver block side
 
 struct TileItem {

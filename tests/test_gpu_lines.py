@@ -19,7 +19,8 @@ def gpu():
 
 
 def db_error(got, want):
-    ok = (want > 0) & (got > 0)
+    assert np.array_equal(want > 0, got > 0), "silent on one side only"
+    ok = want > 0
     return np.abs(10 * np.log10(got[ok] / want[ok]))
 
 
@@ -56,4 +57,20 @@ def test_line_pairs_match_the_oracle(gpu):
     print(f"point bands (summed): n={err_p.size} max={err_p.max():.5f} dB", flush=True)
     assert err_p.max() <= 0.05
     del ref
+    square.close()
+
+
+def test_a_piece_layer_out_of_range_fails_the_call(gpu):
+    from quietmap_amd import scenes
+    from quietmap_amd.painter import PainterError
+
+    print("case: line piece layer out of range", flush=True)
+    sc = scenes.line_scene()
+    pieces = sc.pieces.copy()
+    pieces["layer"][1] = 6
+    square = gpu.upload(sc.scene, sc.sources)
+    pp = np.zeros(2, np.uint32)
+    pl = np.array([0, 1], np.uint32)
+    with pytest.raises(PainterError, match="layer out of range"):
+        gpu.evaluate_line_pairs(square, sc.points, pieces, pp, pl, point_bands=True)
     square.close()

@@ -42,7 +42,9 @@ def db(x):
 
 
 def report(name, err):
-    err = np.abs(err[np.isfinite(err)])
+    err = np.asarray(err)
+    assert not np.isinf(err).any(), f"{name}: silent on one side only"
+    err = np.abs(err[~np.isnan(err)])
     print(f"{name}: n={err.size} median={np.median(err):.5f} p99.9={np.quantile(err, 0.999):.5f} "
           f"max={err.max():.5f} dB", flush=True)
     return err.max()
