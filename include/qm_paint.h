@@ -71,6 +71,18 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
                          double cutoff_m, double* d_period_energy, uint32_t* d_paths, uint32_t* d_failures,
                          void* stream, char* err, size_t errlen);
 
+/* No counterpart in the reference. qm_paint_grid_device for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. It
+ * allocates nothing: the receiver records and queue heads live in a static device array of the
+ * library (QM_NOALLOC_RECEIVERS records of 256 bytes, 16 MiB, plus 256 bytes), one per device,
+ * held until the process ends. More than QM_NOALLOC_RECEIVERS receivers (nx*ny) fail the call.
+ * Every call on a device shares that device's array, so calls that overlap on one device race:
+ * order them on one stream (or with events). Enqueued on `stream`; returns after enqueueing. */
+#define QM_NOALLOC_RECEIVERS 65536
+int qm_paint_grid_device_noalloc(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, uint32_t nx,
+                                 uint32_t ny, double cutoff_m, double* d_period_energy, uint32_t* d_paths,
+                                 uint32_t* d_failures, void* stream, char* err, size_t errlen);
+
 /* The two halves of qm_paint_grid_device. Receiver properties of points on the device: ground
  * altitude + 4 m, air absorption and favourable percentages (the reference's point_receivers,
  * paint-gpu/kernels/paint.cu:62, over paint/src/exact.rs:95-118). */

@@ -43,6 +43,15 @@ struct GridArgs {
     uint32_t* counters;  // XCDS queue heads
 };
 
+// The allocation-free grid painter's memory (qm_paint_grid_device_noalloc): the XCDS queue heads
+// in the first NOALLOC_HEAD_BYTES, then up to QM_NOALLOC_RECEIVERS receiver records. Each device
+// holds its own copy until the process ends. Host code never passes this name to a kernel (on the
+// host it is a stand-in, not a device address): it uses the address hipGetSymbolAddress returned
+// on that device (qm_gpu::noalloc_scratch).
+constexpr size_t NOALLOC_HEAD_BYTES = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD_BYTES + sizeof(qm_receiver) * QM_NOALLOC_RECEIVERS];
+static_assert(sizeof(uint32_t) * XCDS <= NOALLOC_HEAD_BYTES, "queue heads fit the scratch head");
+
 }  // namespace
 
 // paint/src/exact.rs:95-118 Point::with_receiver_properties (receiver_of in qm_eval.hpp).
@@ -257,6 +266,25 @@ int check_device(qm_gpu* gpu, char* err, size_t errlen) {
     return 0;
 }
 
+// The device address of `scratch` on each device, looked up once per device (with that device
+// current) and kept until the process ends.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_address[MAX_DEVICES];
+
+int scratch_address(int ordinal, void** out, char* err, size_t errlen) {
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (ordinal < 0 || ordinal >= MAX_DEVICES) return fail(err, errlen, "device ordinal %d beyond %d", ordinal, MAX_DEVICES);
+    if (!g_scratch_address[ordinal]) {
+        void* p = nullptr;
+        QM_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(err, errlen, "hipGetSymbolAddress returned no address for the painter's scratch");
+        g_scratch_address[ordinal] = p;
+    }
+    *out = g_scratch_address[ordinal];
+    return 0;
+}
+
 // Device memory freed on the stream at scope end (hipMallocAsync's pool).
 struct AsyncBuffer {
     void* p = nullptr;
@@ -319,9 +347,12 @@ int qm_gpu_new(int ordinal, qm_gpu** out, char* err, size_t errlen) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(err, errlen, "GPU %d is %s; libqmpaint is built for gfx950", ordinal, prop.gcnArchName);
     QM_TRY(hipSetDevice(ordinal));
+    void* noalloc_scratch = nullptr;
+    if (scratch_address(ordinal, &noalloc_scratch, err, errlen)) return 1;
     qm_gpu* gpu = new (std::nothrow) qm_gpu();
     if (!gpu) return fail(err, errlen, "out of host memory");
     gpu->ordinal = ordinal;
+    gpu->noalloc_scratch = noalloc_scratch;
     std::snprintf(gpu->name, sizeof(gpu->name), "%s (%s)", prop.name, prop.gcnArchName);
     gpu->bands = make_band_table();
     *out = gpu;
@@ -576,6 +607,27 @@ int qm_paint_grid_device(qm_gpu* gpu, const qm_square* square, const qm_point* d
     if (prepare_receivers(square, d_points, n, (qm_receiver*)receivers.p, stream, err, errlen)) return 1;
     return launch_paint(gpu, square, (const qm_receiver*)receivers.p, nx, ny, cutoff_m, d_period_energy, d_paths,
                         d_failures, (uint32_t*)counters.p, stream, err, errlen);
+}
+
+int qm_paint_grid_device_noalloc(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, uint32_t nx,
+                                 uint32_t ny, double cutoff_m, double* d_period_energy, uint32_t* d_paths,
+                                 uint32_t* d_failures, void* stream_ptr, char* err, size_t errlen) {
+    if (!gpu || !square || !d_points || !d_period_energy || !d_paths || !d_failures)
+        return fail(err, errlen, "null argument");
+    if (!(cutoff_m >= 0.0)) return fail(err, errlen, "cutoff must be non-negative");
+    const size_t n = (size_t)nx * ny;
+    if (n == 0) return 0;
+    if (n > (size_t)QM_NOALLOC_RECEIVERS)
+        return fail(err, errlen, "%zu receivers exceed the allocation-free painter's %d", n, QM_NOALLOC_RECEIVERS);
+    if (!gpu->noalloc_scratch) return fail(err, errlen, "no scratch address for device %d", gpu->ordinal);
+    if (check_device(gpu, err, errlen)) return 1;
+    hipStream_t stream = (hipStream_t)stream_ptr;
+    unsigned char* base = static_cast<unsigned char*>(gpu->noalloc_scratch);
+    uint32_t* d_counters = reinterpret_cast<uint32_t*>(base);
+    qm_receiver* d_receivers = reinterpret_cast<qm_receiver*>(base + NOALLOC_HEAD_BYTES);
+    if (prepare_receivers(square, d_points, n, d_receivers, stream, err, errlen)) return 1;
+    return launch_paint(gpu, square, d_receivers, nx, ny, cutoff_m, d_period_energy, d_paths, d_failures, d_counters,
+                        stream, err, errlen);
 }
 
 int qm_receivers_device(qm_gpu* gpu, const qm_square* square, const qm_point* d_points, size_t n,

@@ -11,6 +11,7 @@ struct qm_gpu {
     char name[256];
     qm::BandTable bands;
     std::mutex lock;
+    void* noalloc_scratch;  // this device's address of painter.hip's static scratch array
 };
 
 // Point sources bucketed on a square grid (device pointers): bucket (kx, ky) holds sources

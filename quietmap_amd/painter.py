@@ -30,6 +30,7 @@ EXPORTS = (
     "qm_evaluate_pairs",
     "qm_paint_grid",
     "qm_paint_grid_device",
+    "qm_paint_grid_device_noalloc",
     "qm_receivers_device",
     "qm_paint_receivers_device",
     "qm_loudness",
@@ -60,6 +61,7 @@ EXPORTS = (
     "qm_tiles_plan_free",
 )
 RECEIVER_BYTES = 256
+NOALLOC_RECEIVERS = 65536  # QM_NOALLOC_RECEIVERS
 
 _lib = None
 
@@ -86,6 +88,7 @@ def library():
         lib.qm_evaluate_pairs.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_paint_grid.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, cp, sz]
         lib.qm_paint_grid_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
+        lib.qm_paint_grid_device_noalloc.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
         lib.qm_receivers_device.argtypes = [vp, vp, vp, sz, vp, vp, cp, sz]
         lib.qm_paint_receivers_device.argtypes = [vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, cp, sz]
         lib.qm_crossings.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp, vp, vp, vp, cp, sz]
@@ -241,6 +244,15 @@ class Gpu:
         err = C.create_string_buffer(512)
         rc = library().qm_paint_grid_device(self.handle, square.handle, d_points, nx, ny, cutoff_m, d_energy,
                                             d_paths, d_failures, stream or None, err, 512)
+        _check(rc, err)
+
+    def paint_grid_device_noalloc(self, square: Square, d_points: int, nx: int, ny: int, cutoff_m: float,
+                                  d_energy: int, d_paths: int, d_failures: int, stream: int = 0):
+        """paint_grid_device without device allocations (the library's static scratch; at most
+        NOALLOC_RECEIVERS receivers; calls overlapping on one device race)."""
+        err = C.create_string_buffer(512)
+        rc = library().qm_paint_grid_device_noalloc(self.handle, square.handle, d_points, nx, ny, cutoff_m, d_energy,
+                                                    d_paths, d_failures, stream or None, err, 512)
         _check(rc, err)
 
     def receivers_device(self, square: Square, d_points: int, n: int, d_receivers: int, stream: int = 0):
