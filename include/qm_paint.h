@@ -135,7 +135,8 @@ int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_cla
  * pixels x 3 periods x QM_LOUD_CURVES x QM_LOUD_BINS x 2 floats, a bin's share of the time and
  * share x mean intensity), and Nden per curve (`ndens`, QM_LOUD_CURVES floats a pixel: each layer
  * alone, then the place). `kurze` is the 41 x 100 quantile table (physics/src/percentile.rs
- * TABLE); null takes qm_kurze_table's. Malformed counts fail the call. Host arrays. */
+ * TABLE); null takes qm_kurze_table's. Malformed counts, and hourly profiles other than none
+ * (-1) and 0-2, fail the call. Host arrays. */
 #define QM_LOUD_BINS 90
 #define QM_LOUD_CURVES 7
 int qm_pixel_loudness(qm_gpu* gpu, const float* input, size_t input_floats, const uint64_t* offsets, size_t pixels,

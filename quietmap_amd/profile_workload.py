@@ -7,7 +7,7 @@ in one paint_kernel launch (the defaults are the benchmark's step), `surrogate` 
 window with the trained network (quietmap_amd/surrogate_params.npy), `loudness` runs
 zwicker_kernel over `--cells` cells of band-major planes, `aircraft` paints the aircraft scene at
 `--aircraft-scale` (`passes` with its 48 pass classes per period), `pixels` runs the pixel
-loudness kernels (mixtures_kernel, ndens_kernel) over `--pixels` synthetic packed pixels,
+loudness kernel (pixel_loudness_kernel) over `--pixels` synthetic packed pixels,
 `flights` sums `--flight-boxes` synthetic aircraft boxes over 20 km at `--flight-points` points
 (every point hearing all 16 tiles: flight_sums_kernel), `horizons` builds the horizons of
 `--horizon-points` receivers of the tile-scale scene at `--scale` 0.25 (horizons_kernel), and

@@ -1,7 +1,9 @@
 """Pixel loudness on the card (level distributions per period, hour group and weather state,
 mixed, then Nden on each curve) against the FP64 oracle on the same packed pixels: Nden within
-0.01 sone per curve, and every mixture's time and energy (the distributions lose none to the grid;
-FP32 atomics and the card's pass tail, 1e-9 of the time, stay within 1e-4)."""
+0.01 sone per curve; every mixture's time and energy within 1e-6 of the oracle's (the
+distributions lose none to the grid; FP32 bins summed from FP64 prefix sums); and bin by bin
+within 2e-3 of the mixture's total (a sum lying within FP32 rounding of a bin edge may fall on the
+other side of it)."""
 import numpy as np
 import pytest
 
@@ -35,8 +37,11 @@ def test_pixel_loudness_matches_oracle(gpu, seed, count, kurze):
         w = wmix[..., k].sum(axis=-1)
         scale = np.maximum(w, 1e-30 if k else 1.0)
         rel = np.abs(g - w) / scale
-        print(f"mixtures {'energy' if k else 'time'}: max relative {rel.max():.2e}", flush=True)
-        assert rel.max() < 1e-4
+        per_bin = np.abs(gmix[..., k].astype(np.float64) - wmix[..., k]).max(axis=-1) / scale
+        print(f"mixtures {'energy' if k else 'time'}: max relative {rel.max():.2e}, bin by bin {per_bin.max():.2e}",
+              flush=True)
+        assert rel.max() < 1e-6
+        assert per_bin.max() < 2e-3
     assert (want[:, 6] > 0).sum() > count // 2
 
 
