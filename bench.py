@@ -25,6 +25,39 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "receiver-source paths/s & Lden cells/s at ≤0.05 dB; surrogate vs exact, 1 GPU"
+# Worker threads of the CPU baseline: the host's cores, at most the CPU share of one GPU's box.
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """The host's CPU model and core count, stated next to every CPU figure."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), model)
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def median_seconds(run, repeats=3):
+    """The median wall time of `repeats` runs of `run()` (warm: the inputs are already built)."""
+    times = []
+    for _ in range(repeats):
+        t = time.time()
+        run()
+        times.append(time.time() - t)
+    return sorted(times)[len(times) // 2]
+
+
+def threaded_chunks(fn, n, threads):
+    """fn(begin, end) over `n` items split into `threads` chunks run on a thread pool (the oracle's C
+    calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    bounds = [(n * k // threads, n * (k + 1) // threads) for k in range(threads)]
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(lambda b: fn(*b), bounds))
 
 
 def main():
@@ -174,14 +207,23 @@ def main():
         if not args.no_cpu:
             from oracle import oracle
 
-            pick = np.arange(0, points_f.size, 64)  # 64 points' segments
+            oracle.use_native()
+            threads = args.cpu_threads or min(os.cpu_count() or 1, CPU_SHARE)
+            pick = np.arange(0, points_f.size, 16)  # 256 points' segments
             keep = np.isin(segments_f["point"], pick)
-            t1 = time.time()
-            want_f = oracle.flight_sums(boxes_f, points_f, None, segments_f[keep], n_sums_f)
-            cpu_f = int(segments_f["count"][keep].astype(np.int64).sum()) / (time.time() - t1)
-            fl["cpu_baseline"] = {"value": cpu_f, "unit": "box reads/s", "cores": 1, "kind": "port",
+            sel = segments_f[keep]
+            want_f = oracle.flight_sums(boxes_f, points_f, None, sel, n_sums_f)
+
+            def flights_cpu(begin, end):
+                oracle.flight_sums(boxes_f, points_f, None, sel[begin:end], n_sums_f)
+
+            reads_cpu = int(sel["count"].astype(np.int64).sum())
+            cpu_f = reads_cpu / median_seconds(lambda: threaded_chunks(flights_cpu, sel.size, threads))
+            one_f = reads_cpu / median_seconds(lambda: flights_cpu(0, sel.size))
+            fl["cpu_baseline"] = {"value": cpu_f, "unit": "box reads/s", "cores": threads, "kind": "port",
+                                  "one_thread": one_f, **host_cpu(), "median_of": 3, "build": oracle.NATIVE_FLAGS,
                                   "sample": f"the {int(keep.sum())} segments of {pick.size} points, FP64 C++ "
-                                            "restatement, one thread"}
+                                            "restatement of the reference's CPU path"}
             e_g, e_w = sums_f[pick, :3], want_f[pick, :3]
             ok = e_w > 0
             fl["max_abs_db_vs_oracle"] = float(np.abs(10 * np.log10(e_g[ok] / e_w[ok])).max())
@@ -216,12 +258,19 @@ def main():
         if not args.no_cpu:
             from oracle import oracle
 
-            k = 20000
-            t1 = time.time()
-            want = oracle.zwicker_sone(planes[:, :k].T.astype(np.float64))
-            cpu_l = k / (time.time() - t1)
-            loud["cpu_baseline"] = {"value": cpu_l, "unit": "cells/s", "cores": 1, "kind": "port",
-                                    "sample": f"{k} cells, FP64 C++ restatement, one thread"}
+            oracle.use_native()
+            threads = args.cpu_threads or min(os.cpu_count() or 1, CPU_SHARE)
+            k = 200000
+            cells64 = np.ascontiguousarray(planes[:, :k].T.astype(np.float64))
+            want = oracle.zwicker_sone(cells64)
+            cpu_l = k / median_seconds(lambda: threaded_chunks(lambda a, b: oracle.zwicker_sone(cells64[a:b]), k,
+                                                               threads))
+            k1 = 50000
+            one_l = k1 / median_seconds(lambda: oracle.zwicker_sone(cells64[:k1]))
+            loud["cpu_baseline"] = {"value": cpu_l, "unit": "cells/s", "cores": threads, "kind": "port",
+                                    "one_thread": one_l, **host_cpu(), "median_of": 3, "build": oracle.NATIVE_FLAGS,
+                                    "sample": f"{k} cells ({k1} on one thread), FP64 C++ restatement of the "
+                                              "reference's CPU path"}
             loud["max_abs_sone_vs_oracle"] = float(np.abs(sone[:k].astype(np.float64) - want).max())
         print(f"[bench] loudness {loud['cells_per_s']:.3e} cells/s", file=sys.stderr, flush=True)
 
@@ -233,21 +282,29 @@ def main():
     if not args.no_cpu:
         from oracle import oracle
 
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        oracle.use_native()
+        threads = args.cpu_threads or min(os.cpu_count() or 1, CPU_SHARE)
         rng = np.random.default_rng(1234)
-        probe = rng.choice(n, 8, replace=False)
+        probe = rng.choice(n, 4 * threads, replace=False)
         t1 = time.time()
         oracle.paint_points(sc.scene, points[probe], sc.sources, sc.cutoff_m, threads=threads)
-        per_receiver = (time.time() - t1) / 8
-        count = int(min(max(args.cpu_seconds / max(per_receiver, 1e-6), 16), 20000))
+        per_receiver = (time.time() - t1) / probe.size
+        # Three runs of about a sixth of --cpu-seconds each on `threads` threads, then as long on one.
+        count = int(min(max(args.cpu_seconds / 6.0 / max(per_receiver, 1e-6), 16), 20000))
         sample = rng.choice(n, count, replace=False)
-        t1 = time.time()
         ref = oracle.paint_points(sc.scene, points[sample], sc.sources, sc.cutoff_m, threads=threads)
-        cpu_s = time.time() - t1
         cpu_paths = int(ref["paths"].sum())
+        cpu_s = median_seconds(lambda: oracle.paint_points(sc.scene, points[sample], sc.sources, sc.cutoff_m,
+                                                           threads=threads))
+        count1 = max(count // threads, 2)
+        paths1 = int(ref["paths"][:count1].sum())
+        one_s = median_seconds(lambda: oracle.paint_points(sc.scene, points[sample[:count1]], sc.sources,
+                                                           sc.cutoff_m, threads=1))
         cpu = {"value": cpu_paths / cpu_s, "unit": "paths/s", "cores": threads, "kind": "port",
+               "one_thread": paths1 / one_s, **host_cpu(), "median_of": 3, "build": oracle.NATIVE_FLAGS,
                "sample": f"{count} random receivers of the workload, all {cpu_paths} paths within 2 km "
-                         f"({cpu_s:.1f} s); FP64 C++ restatement of the reference's Rust CPU path"}
+                         f"({cpu_s:.1f} s a run; the first {count1} on one thread); FP64 C++ restatement of "
+                         "the reference's Rust CPU path (Rust toolchain unavailable)"}
         got = lden_db(energy[sample])
         want = lden_db(ref["period_energy"])
         err = np.abs(got - want)

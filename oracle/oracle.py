@@ -23,6 +23,20 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
 
 
+NATIVE = os.path.join(HERE, "_native", "liboracle.so")
+NATIVE_FLAGS = "g++ -O3 -march=native -ffp-contract=off"
+
+
+def use_native():
+    """Load the CPU baseline's build (NATIVE_FLAGS, compiled on this host for its own CPU) in place of
+    the checker build; both compute the same (no fast math, no contraction)."""
+    global LIBRARY, _lib
+    if LIBRARY == NATIVE and _lib is not None:
+        return
+    subprocess.run(["make", "-s", "-j8", "-C", HERE, "_native/liboracle.so"], check=True)
+    LIBRARY, _lib = NATIVE, None
+
+
 def lib():
     global _lib
     if _lib is None:
