@@ -3,7 +3,10 @@
 //
 //   line_nodes_kernel - per pair: the piece's quadrature nodes seen from the receiver (qm_line.hpp:
 //                       FP64 geometry, the skyline mask of wide buckets), at most LINE_NODE_CAP
-//   (host)            - exclusive scan of the node counts: one ray per node, pair-major
+//   scan_blocks_kernel, scan_offsets_kernel, list_rays_kernel
+//                     - the exclusive scan of the node counts on the device (each block's 1,024
+//                       counts by wave scans, then the blocks' offsets), and each pair's rays
+//                       listed from its first: one ray per node, pair-major
 //   line_rays_kernel  - per ray: the exact path from the node's point on the piece to the
 //                       receiver, weighted by its dphi and the piece's divergence factor, walls
 //                       searched unless the node stands in a clear gap; the pair's 9 sums and the
@@ -156,6 +159,81 @@ __global__ __launch_bounds__(256) void line_receivers_kernel(Scene sc, const qm_
     if (i < n) out[i] = receiver_of(sc, points[i]);
 }
 
+constexpr int SCAN_GROUP = 256;
+constexpr int SCAN_PER_THREAD = 4;
+constexpr int SCAN_BLOCK = SCAN_GROUP * SCAN_PER_THREAD;
+
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t x, int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// Per block of SCAN_BLOCK counts: each count's exclusive prefix within the block (into `first`)
+// and the block's total.
+__global__ __launch_bounds__(SCAN_GROUP) void scan_blocks_kernel(const uint32_t* counts, uint32_t n, uint32_t* first,
+                                                                 uint32_t* block_total) {
+    __shared__ uint32_t wave_total[SCAN_GROUP / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * SCAN_BLOCK + (size_t)tid * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD], mine = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        v[k] = base + k < n ? counts[base + k] : 0u;
+        mine += v[k];
+    }
+    const uint32_t inclusive = wave_inclusive(mine, lane);
+    if (lane == 63) wave_total[wave] = inclusive;
+    __syncthreads();
+    uint32_t before = inclusive - mine;
+    for (int w = 0; w < wave; ++w) before += wave_total[w];
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        if (base + k < n) first[base + k] = before;
+        before += v[k];
+    }
+    if (tid == SCAN_GROUP - 1) block_total[blockIdx.x] = before;
+}
+
+// One workgroup: the blocks' exclusive offsets (64-bit, in chunks of SCAN_GROUP blocks with a
+// carry) and the grand total.
+__global__ __launch_bounds__(SCAN_GROUP) void scan_offsets_kernel(const uint32_t* block_total, uint32_t blocks,
+                                                                  uint64_t* block_first, uint64_t* total) {
+    __shared__ uint64_t wave_total[SCAN_GROUP / 64];
+    __shared__ uint64_t carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t at = 0; at < blocks; at += SCAN_GROUP) {
+        const uint64_t mine = at + tid < blocks ? block_total[at + tid] : 0ull;
+        uint64_t inclusive = mine;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t y = __shfl_up(inclusive, d);
+            if (lane >= d) inclusive += y;
+        }
+        if (lane == 63) wave_total[wave] = inclusive;
+        __syncthreads();
+        uint64_t before = carry + inclusive - mine;
+        for (int w = 0; w < wave; ++w) before += wave_total[w];
+        if (at + tid < blocks) block_first[at + tid] = before;
+        __syncthreads();
+        if (tid == SCAN_GROUP - 1) carry = before + mine;
+        __syncthreads();
+    }
+    if (tid == 0) *total = carry;
+}
+
+// Each pair's first ray (its block's offset added) and the pair of each of its rays.
+__global__ __launch_bounds__(256) void list_rays_kernel(const uint32_t* counts, uint32_t n,
+                                                        const uint64_t* block_first, uint32_t* first,
+                                                        uint32_t* ray_pair) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = (uint32_t)(block_first[i / SCAN_BLOCK] + first[i]);
+    first[i] = f;
+    for (uint32_t k = 0; k < counts[i]; ++k) ray_pair[f + k] = i;
+}
+
 // The point bands of every pair that did not fail (a failed pair goes to the caller's fallback
 // whole, as its sums do).
 __global__ __launch_bounds__(256) void line_bands_kernel(const uint32_t* pair_point, const qm_line_piece* pieces,
@@ -224,21 +302,27 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     hipLaunchKernelGGL(line_nodes_kernel, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, 0, square->scene,
                        d_receivers, d_pieces, d_pp, d_pl, (uint32_t)n_pairs, d_nodes, d_counts, d_div, d_failed);
     L_TRY(hipGetLastError());
-    // One ray per node, pair-major (the reference's count / scan / list passes).
-    std::vector<uint32_t> counts(n_pairs), first(n_pairs);
-    L_TRY(hipMemcpy(counts.data(), d_counts, 4 * n_pairs, hipMemcpyDeviceToHost));
+    // One ray per node, pair-major (the reference's count / scan / list passes, on the device; only
+    // the total comes back, to size the ray list).
+    const uint32_t blocks = (uint32_t)((n_pairs + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    uint32_t* d_block_total;
+    uint64_t *d_block_first, *d_total;
+    L_TRY(b.alloc(&d_block_total, blocks));
+    L_TRY(b.alloc(&d_block_first, blocks));
+    L_TRY(b.alloc(&d_total, 1));
+    hipLaunchKernelGGL(scan_blocks_kernel, dim3(blocks), dim3(SCAN_GROUP), 0, 0, (const uint32_t*)d_counts,
+                       (uint32_t)n_pairs, d_ray_first, d_block_total);
+    hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(SCAN_GROUP), 0, 0, (const uint32_t*)d_block_total, blocks,
+                       d_block_first, d_total);
+    L_TRY(hipGetLastError());
     uint64_t total = 0;
-    for (size_t i = 0; i < n_pairs; ++i) {
-        first[i] = (uint32_t)total;
-        total += counts[i];
-    }
+    L_TRY(hipMemcpy(&total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (total > 0xffffffffu) return fail(err, errlen, "too many rays");
-    std::vector<uint32_t> ray_pair(total);
-    for (size_t i = 0; i < n_pairs; ++i)
-        for (uint32_t k = 0; k < counts[i]; ++k) ray_pair[first[i] + k] = (uint32_t)i;
     L_TRY(b.alloc(&d_ray_pair, total));
-    if (total) L_TRY(hipMemcpy(d_ray_pair, ray_pair.data(), 4 * total, hipMemcpyHostToDevice));
-    L_TRY(hipMemcpy(d_ray_first, first.data(), 4 * n_pairs, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(list_rays_kernel, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, 0,
+                       (const uint32_t*)d_counts, (uint32_t)n_pairs, (const uint64_t*)d_block_first, d_ray_first,
+                       d_ray_pair);
+    L_TRY(hipGetLastError());
     if (total) {
         if (square->scene.terrain.fine_detail)
             hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
@@ -261,8 +345,11 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     for (size_t i = 0; i < n_pairs; ++i)
         if (failed[i])
             for (int j = 0; j < QM_PAIR_SUMS; ++j) sums[i * QM_PAIR_SUMS + j] = 0.0f;
-    if (rays)
-        for (size_t i = 0; i < n_pairs; ++i) rays[i] = failed[i] ? 0u : counts[i];
+    if (rays) {
+        L_TRY(hipMemcpy(rays, d_counts, 4 * n_pairs, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n_pairs; ++i)
+            if (failed[i]) rays[i] = 0u;
+    }
     if (point_bands) {
         L_TRY(hipMemcpy(point_bands, d_bands, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS,
                         hipMemcpyDeviceToHost));

@@ -74,3 +74,26 @@ def test_a_piece_layer_out_of_range_fails_the_call(gpu):
     with pytest.raises(PainterError, match="layer out of range"):
         gpu.evaluate_line_pairs(square, sc.points, pieces, pp, pl, point_bands=True)
     square.close()
+
+
+def test_ray_scan_over_many_blocks(gpu):
+    """300,000 pairs (the same 3,000 a hundred times): the device scan of their ray counts spans
+    293 blocks of 1,024 and two chunks of block offsets; every copy lists its rays where the first
+    copy does, so its rays and sums equal the first copy's (sums to FP32 atomics' order)."""
+    from quietmap_amd import scenes
+
+    print("case: line ray scan over many blocks", flush=True)
+    sc = scenes.line_scene()
+    rng = np.random.default_rng(8)
+    n, copies = 3000, 100
+    pp = rng.integers(0, sc.points.size, n).astype(np.uint32)
+    pl = rng.integers(0, sc.pieces.size, n).astype(np.uint32)
+    square = gpu.upload(sc.scene, sc.sources)
+    got = gpu.evaluate_line_pairs(square, sc.points, sc.pieces, np.tile(pp, copies), np.tile(pl, copies))
+    rays = got["rays"].reshape(copies, n)
+    sums = got["sums"].reshape(copies, n, -1).astype(np.float64)
+    failed = got["failed"].reshape(copies, n)
+    print(f"rays {int(rays.sum())}", flush=True)
+    assert (rays == rays[0]).all() and (failed == failed[0]).all()
+    assert np.allclose(sums, sums[0][None], rtol=1e-5, atol=0.0)
+    square.close()
