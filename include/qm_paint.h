@@ -197,6 +197,36 @@ int qm_crossings(qm_gpu* gpu, const qm_square* square, const double* segments, s
                  uint32_t capacity, double* t, uint64_t* footprint, uint32_t* outline, uint32_t* edge,
                  uint32_t* flags, uint32_t* count, uint32_t* failed, char* err, size_t errlen);
 
+/* Raster-out: a painted square's quantized cells (paint/src/levels.rs:58-109 PaintedCells::
+ * from_painted): per pixel (`assessed` 0: 255 in every cell) the Nden cell of each layer's and
+ * the place's loudness (`ndens`, 7 sone a pixel; levels.rs cell: 80 + 20 log2 sone, half a phon a
+ * step, 254 below 0 phon) and the Lden cell of each layer's energy and of their sum (`layer_energy`,
+ * 6 linear Lden energies a pixel; levels.rs lden_cell: twice Lden in dB, 254 below 0 dB), 7 bytes a
+ * pixel each. A negative or non-finite energy fails the call naming the pixel and layer, as
+ * lden_cell's Err does. Host arrays. */
+int qm_level_cells(qm_gpu* gpu, const uint8_t* assessed, const double* layer_energy, const double* ndens,
+                   size_t pixels, uint8_t* nden_cells, uint8_t* lden_cells, char* err, size_t errlen);
+/* The same on device arrays, enqueued on `stream`: the lowest pixel x 7 + cell of an invalid energy
+ * is atomically min-ed into `d_first_bad` (the caller sets it to ~0 first). */
+int qm_level_cells_device(qm_gpu* gpu, const uint8_t* d_assessed, const double* d_layer_energy, const double* d_ndens,
+                          size_t pixels, uint8_t* d_nden_cells, uint8_t* d_lden_cells, unsigned long long* d_first_bad,
+                          void* stream, char* err, size_t errlen);
+
+/* HM3 tiles before Brotli (paint/src/hm3.rs:68-108 encode_metric_tile; Brotli stays on the host):
+ * per 512 x 512 tile of cells (row-major, tile after tile) the metric's 5-byte header ("HM3 \x07"
+ * for Nden, "HMD \x01" for Lden), the tile's layer byte (`tile_layer`, 0-6) and each cell minus
+ * PNG's Paeth predictor of it (filter 4 with its tie order), QM_HM3_PREDICTED_BYTES a tile. Host
+ * arrays; a layer above 6 fails the call. */
+#define QM_HM3_TILE_PX 512
+#define QM_HM3_PREDICTED_BYTES (6 + QM_HM3_TILE_PX * QM_HM3_TILE_PX)
+#define QM_HM3_NDEN 0
+#define QM_HM3_LDEN 1
+int qm_hm3_predict(qm_gpu* gpu, const uint8_t* cells, const uint8_t* tile_layer, size_t tiles, int metric,
+                   uint8_t* out, char* err, size_t errlen);
+/* The same on device arrays, enqueued on `stream` (layers unchecked: the caller's). */
+int qm_hm3_predict_device(qm_gpu* gpu, const uint8_t* d_cells, const uint8_t* d_tile_layer, size_t tiles, int metric,
+                          uint8_t* d_out, void* stream, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

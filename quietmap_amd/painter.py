@@ -54,6 +54,10 @@ EXPORTS = (
     "qm_mlp_infer_device",
     "qm_surrogate_samples_device",
     "qm_surrogate_paint_receivers_device",
+    "qm_level_cells",
+    "qm_level_cells_device",
+    "qm_hm3_predict",
+    "qm_hm3_predict_device",
     "qm_tiles_plan",
     "qm_tiles_run",
     "qm_tiles_paths",
@@ -117,6 +121,10 @@ def library():
         lib.qm_mlp_infer_device.argtypes = [vp, vp, sz, vp, vp, cp, sz]
         lib.qm_surrogate_samples_device.argtypes = [vp, vp, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_surrogate_paint_receivers_device.argtypes = [vp, vp, vp, vp, u32, u32, dbl, vp, vp, vp, vp, vp, cp, sz]
+        lib.qm_level_cells.argtypes = [vp, vp, vp, vp, sz, vp, vp, cp, sz]
+        lib.qm_level_cells_device.argtypes = [vp, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
+        lib.qm_hm3_predict.argtypes = [vp, vp, vp, sz, C.c_int, vp, cp, sz]
+        lib.qm_hm3_predict_device.argtypes = [vp, vp, vp, sz, C.c_int, vp, vp, cp, sz]
         lib.qm_tiles_plan.argtypes = [vp, vp, vp, sz, vp, sz, u32, u32, vp, vp, vp, C.POINTER(vp), cp, sz]
         lib.qm_tiles_run.argtypes = [vp, vp, cp, sz]
         lib.qm_tiles_paths.argtypes = [vp]
@@ -402,6 +410,32 @@ class Gpu:
                                               _ptr(bands), _ptr(spectra), err, 512)
         _check(rc, err)
         return dict(sums=sums, failed=failed, rays=rays, point_bands=bands, pair_bands=spectra)
+
+    def level_cells(self, assessed, layer_energy, ndens):
+        """PaintedCells::from_painted: per pixel its Nden and Lden cells (pixels, 7) uint8 each."""
+        assessed = np.ascontiguousarray(assessed, np.uint8).reshape(-1)
+        n = assessed.size
+        energy = np.ascontiguousarray(layer_energy, np.float64).reshape(n, 6)
+        nd = np.ascontiguousarray(ndens, np.float64).reshape(n, 7)
+        nden = np.zeros((n, 7), np.uint8)
+        lden = np.zeros((n, 7), np.uint8)
+        err = C.create_string_buffer(512)
+        rc = library().qm_level_cells(self.handle, _ptr(assessed), _ptr(energy), _ptr(nd), n, _ptr(nden), _ptr(lden),
+                                      err, 512)
+        _check(rc, err)
+        return nden, lden
+
+    def hm3_predict(self, cells, tile_layer, metric="nden"):
+        """encode_metric_tile before Brotli for tiles of 512 x 512 cells: (tiles, 6 + 512 * 512) bytes."""
+        cells = np.ascontiguousarray(cells, np.uint8).reshape(-1, 512 * 512)
+        layers = np.ascontiguousarray(tile_layer, np.uint8).reshape(-1)
+        assert layers.size == cells.shape[0]
+        out = np.zeros((cells.shape[0], 6 + 512 * 512), np.uint8)
+        err = C.create_string_buffer(512)
+        rc = library().qm_hm3_predict(self.handle, _ptr(cells), _ptr(layers), layers.size,
+                                      0 if metric == "nden" else 1, _ptr(out), err, 512)
+        _check(rc, err)
+        return out
 
     def crossings(self, square: Square, segments, capacity: int = 48):
         seg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)

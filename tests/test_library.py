@@ -15,7 +15,7 @@ def declared():
     names = set()
     for header in ("qm_paint.h", "qm_surrogate.h", "qm_tiles.h"):
         text = open(os.path.join(ROOT, "include", header)).read()
-        names |= set(re.findall(r"\b(qm_[a-z_]+)\s*\(", text))
+        names |= set(re.findall(r"\b(qm_[a-z0-9_]+)\s*\(", text))
     return sorted(names)
 
 
