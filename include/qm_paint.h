@@ -142,6 +142,22 @@ int qm_aircraft_grid(qm_gpu* gpu, const qm_aircraft_class* classes, size_t n_cla
 int qm_pixel_loudness(qm_gpu* gpu, const float* input, size_t input_floats, const uint64_t* offsets, size_t pixels,
                       const float* kurze, float* mixtures, float* ndens, char* err, size_t errlen);
 
+/* The map's folding and packing of pixel sound on the card, then qm_pixel_loudness of the result
+ * (the reference folds on the cores before its upload: paint/src/sone.rs:268-276 PixelSound::pack
+ * over paint/src/sone_fold.rs:115-405 fold_placed_for_map, whose end is
+ * popup/src/percentiles/weather.rs:139-175 merge_close_states at the map's spread 10^0.15, then
+ * write_packed, sone.rs:217-234). `unfolded` holds pixels in qm_pixel_loudness's layout with every
+ * value FP64 and the lines as placed: per layer, the weak events fold into one steady line within
+ * their variance budget, the Kurze lines that never exceed 5 % of the held level into a steady line
+ * per hourly profile, the passes that hold their mean likewise; the weather states each line keeps
+ * within the spread merge; a layer's first steady line whose states read alike in FP32 joins its
+ * steady energy. The folded pixel (FP32, at the pixel's offset, never longer than the unfolded one)
+ * goes to `folded` when given (input_floats floats; NaN past each pixel's end), and on the card
+ * into the loudness kernel; `mixtures` and `ndens` as qm_pixel_loudness's. Host arrays. */
+int qm_pixel_loudness_unfolded(qm_gpu* gpu, const double* unfolded, size_t input_floats, const uint64_t* offsets,
+                               size_t pixels, const float* kurze, float* mixtures, float* ndens, float* folded,
+                               char* err, size_t errlen);
+
 /* The quantile table of Kurze's line statistics (physics/src/percentile_table.rs): the
  * reference's seeded simulation (physics/src/percentile_tests.rs writes_the_table) restated, 41
  * rows of lambda from 1e-4 at 8 a decade by 100 quantiles, FP32. */

@@ -649,10 +649,419 @@ double nden(const Distribution* per_period, Curve* curves) {
     return total / 24.0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// paint/src/sone_fold.rs fold_placed_for_map and popup/src/percentiles/weather.rs
+// merge_close_states, over lines in f64 (popup/src/percentiles/lines.rs Line, Line::steady), then
+// paint/src/sone.rs write_packed / pack_line into the card's f32 layout.
+
+constexpr double DUTY_FOLD_PEAK_SHARE = 0.12;
+constexpr double DUTY_FOLD_STD_SHARE = 0.08;
+constexpr double KURZE_FOLD_HELD_SHARE = 0.05;
+constexpr int MAX_GROUPS = 4;
+
+struct FLine {
+    double energy[PERIODS];
+    double means[PERIODS][STATES];
+    double lambda[PERIODS];
+    int profile;  // -1 none
+    bool has_duty, has_passing;
+    double duty[PERIODS], passing[PERIODS];
+
+    static FLine unpack(const double* f) {
+        FLine l;
+        for (int p = 0; p < PERIODS; ++p) l.energy[p] = f[p];
+        for (int p = 0; p < PERIODS; ++p)
+            for (int s = 0; s < STATES; ++s) l.means[p][s] = f[PERIODS + p * STATES + s];
+        for (int p = 0; p < PERIODS; ++p) l.lambda[p] = f[PERIODS + PERIODS * STATES + p];
+        l.profile = (int)f[2 * PERIODS + PERIODS * STATES];
+        l.has_duty = !std::isnan(f[2 * PERIODS + PERIODS * STATES + 1]);
+        l.has_passing = !std::isnan(f[3 * PERIODS + PERIODS * STATES + 1]);
+        for (int p = 0; p < PERIODS; ++p) {
+            l.duty[p] = f[2 * PERIODS + PERIODS * STATES + 1 + p];
+            l.passing[p] = f[3 * PERIODS + PERIODS * STATES + 1 + p];
+        }
+        return l;
+    }
+    // Line::steady
+    static FLine steady(const double means[PERIODS][STATES], int profile) {
+        FLine l;
+        for (int p = 0; p < PERIODS; ++p) {
+            bool any = false;
+            for (int s = 0; s < STATES; ++s) {
+                l.means[p][s] = means[p][s];
+                any = any || means[p][s] > 0.0;
+            }
+            l.energy[p] = any ? 1.0 : 0.0;
+            l.lambda[p] = std::numeric_limits<double>::infinity();
+            l.duty[p] = l.passing[p] = std::nan("");
+        }
+        l.profile = profile;
+        l.has_duty = l.has_passing = false;
+        return l;
+    }
+    template <typename V>
+    void pack(V* out) const {
+        for (int p = 0; p < PERIODS; ++p) out[p] = (V)energy[p];
+        for (int p = 0; p < PERIODS; ++p)
+            for (int s = 0; s < STATES; ++s) out[PERIODS + p * STATES + s] = (V)means[p][s];
+        for (int p = 0; p < PERIODS; ++p) out[PERIODS + PERIODS * STATES + p] = (V)lambda[p];
+        out[2 * PERIODS + PERIODS * STATES] = (V)profile;
+        for (int p = 0; p < PERIODS; ++p) {
+            out[2 * PERIODS + PERIODS * STATES + 1 + p] = has_duty ? (V)duty[p] : (V)std::nan("");
+            out[3 * PERIODS + PERIODS * STATES + 1 + p] = has_passing ? (V)passing[p] : (V)std::nan("");
+        }
+    }
+};
+
+struct FLayer {
+    std::vector<FLine> lines;
+    double steady[PERIODS];
+};
+
+// The map's hour groups (count per period) and each profile's mean factor per group.
+struct MapHours {
+    int groups[PERIODS];
+    double factor[PROFILES][PERIODS][MAX_GROUPS];
+};
+const MapHours& map_hours() {
+    static MapHours h;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int p = 0; p < PERIODS; ++p) {
+            int hours = 0;
+            const auto g = hour_groups(true, p, hours);
+            h.groups[p] = (int)g.size();
+            for (int profile = 0; profile < PROFILES; ++profile)
+                for (int k = 0; k < MAX_GROUPS; ++k) {
+                    double f = 1.0;
+                    if (k < (int)g.size()) {
+                        f = 0.0;
+                        for (int slot : g[k]) f += hour_factor(profile, p, slot);
+                        f /= (double)g[k].size();
+                    }
+                    h.factor[profile][p][k] = f;
+                }
+        }
+    });
+    return h;
+}
+
+void merge_close_states(std::vector<FLayer>& layers, double spread) {
+    auto within = [&](const double* means, int n) {
+        double low = std::numeric_limits<double>::infinity(), high = 0.0;
+        for (int k = 0; k < n; ++k) {
+            if (!std::isfinite(means[k])) return false;
+            low = std::min(low, means[k]);
+            high = std::max(high, means[k]);
+        }
+        return high <= spread * low;
+    };
+    for (int p = 0; p < PERIODS; ++p) {
+        int start = 0;
+        while (start < STATES) {
+            int end = start + 1;
+            for (;;) {
+                if (end >= STATES) break;
+                bool all = true;
+                for (const FLayer& layer : layers)
+                    for (const FLine& line : layer.lines) all = all && within(&line.means[p][start], end - start + 1);
+                if (!all) break;
+                ++end;
+            }
+            if (end - start > 1)
+                for (FLayer& layer : layers)
+                    for (FLine& line : layer.lines) {
+                        double* run = &line.means[p][start];
+                        bool differs = false;
+                        for (int k = 1; k < end - start; ++k) differs = differs || run[k] != run[0];
+                        if (differs) {
+                            double sum = 0.0;
+                            for (int k = 0; k < end - start; ++k) sum += run[k];
+                            const double mean = sum / (double)(end - start);
+                            for (int k = 0; k < end - start; ++k) run[k] = mean;
+                        }
+                    }
+            start = end;
+        }
+    }
+}
+
+void fold_placed_for_map(std::vector<FLayer>& placed, double spread) {
+    const MapHours& hours = map_hours();
+    const float* table = kurze_table().data();
+    bool any_profiled = false;
+    for (const FLayer& layer : placed)
+        for (const FLine& line : layer.lines) any_profiled = any_profiled || line.profile >= 0;
+    for (FLayer& layer : placed) {
+        std::vector<FLine>& lines = layer.lines;
+        if (lines.empty()) continue;
+        bool has_kurze = false, has_duty = false;
+        for (const FLine& line : lines) {
+            if (line.has_duty) {
+                has_duty = true;
+            } else if (!line.has_passing) {
+                for (int p = 0; p < PERIODS; ++p) has_kurze = has_kurze || std::isfinite(line.lambda[p]);
+            }
+        }
+        double held[PERIODS][MAX_GROUPS][STATES] = {};
+        if (has_kurze) {
+            for (int p = 0; p < PERIODS; ++p) {
+                const int ng = any_profiled ? hours.groups[p] : 1;
+                for (int g = 0; g < ng; ++g)
+                    for (int s = 0; s < STATES; ++s) held[p][g][s] = layer.steady[p];
+            }
+            for (const FLine& line : lines) {
+                if (line.has_duty || line.has_passing) continue;
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line.energy[p] <= 0.0) continue;
+                    const int ng = any_profiled ? hours.groups[p] : 1;
+                    for (int g = 0; g < ng; ++g) {
+                        const double factor = any_profiled && line.profile >= 0 ? hours.factor[line.profile][p][g] : 1.0;
+                        for (int s = 0; s < STATES; ++s) {
+                            const double m = line.means[p][s] * factor;
+                            if (m > 0.0) held[p][g][s] += m;
+                        }
+                    }
+                }
+            }
+        }
+        double layer_total[PERIODS][STATES] = {};
+        if (has_duty) {
+            for (int p = 0; p < PERIODS; ++p)
+                for (int s = 0; s < STATES; ++s) layer_total[p][s] = layer.steady[p];
+            for (const FLine& line : lines)
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line.energy[p] <= 0.0) continue;
+                    for (int s = 0; s < STATES; ++s)
+                        if (line.means[p][s] > 0.0) layer_total[p][s] += line.means[p][s];
+                }
+        }
+        double unprof[PERIODS][STATES] = {};
+        bool has_unprof = false;
+        double prof[PROFILES][PERIODS][STATES] = {};
+        bool has_prof[PROFILES] = {false, false, false};
+        std::vector<bool> keep(lines.size(), false);
+        std::vector<std::pair<double, size_t>> duty_candidates;
+        for (size_t idx = 0; idx < lines.size(); ++idx) {
+            const FLine& line = lines[idx];
+            if (line.has_duty) {
+                bool moves = false, eligible = true;
+                double max_var_ratio = 0.0;
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line.energy[p] <= 0.0 || line.duty[p] <= 0.0) continue;
+                    const double d = line.duty[p];
+                    const double var_scale = std::max(1.0 - d, 0.0) / d;
+                    for (int s = 0; s < STATES; ++s) {
+                        const double mean = line.means[p][s];
+                        if (mean <= 0.0) continue;
+                        moves = true;
+                        const double total = layer_total[p][s];
+                        if (total <= 0.0 || mean / d > DUTY_FOLD_PEAK_SHARE * total) {
+                            eligible = false;
+                        } else {
+                            const double r = mean / total;
+                            max_var_ratio = std::max(max_var_ratio, r * r * var_scale);
+                        }
+                    }
+                }
+                if (moves) {
+                    if (eligible) duty_candidates.push_back({max_var_ratio, idx});
+                    else keep[idx] = true;
+                }
+                continue;
+            }
+            if (line.has_passing) {
+                bool moves = false;
+                for (int p = 0; p < PERIODS; ++p) {
+                    bool any = false;
+                    for (int s = 0; s < STATES; ++s) any = any || line.means[p][s] > 0.0;
+                    moves = moves || (line.energy[p] > 0.0 && line.passing[p] > 0.0 &&
+                                      line.passing[p] < PASSES_HOLD_MEAN && any);
+                }
+                if (moves) {
+                    keep[idx] = true;
+                } else {
+                    for (int p = 0; p < PERIODS; ++p) {
+                        if (line.energy[p] <= 0.0) continue;
+                        for (int s = 0; s < STATES; ++s)
+                            if (line.means[p][s] > 0.0) {
+                                unprof[p][s] += line.means[p][s];
+                                has_unprof = true;
+                            }
+                    }
+                }
+                continue;
+            }
+            bool always_held = true;
+            for (int p = 0; p < PERIODS && always_held; ++p) {
+                if (line.energy[p] <= 0.0) continue;
+                const int ng = any_profiled ? hours.groups[p] : 1;
+                for (int g = 0; g < ng && always_held; ++g) {
+                    const double factor = any_profiled && line.profile >= 0 ? hours.factor[line.profile][p][g] : 1.0;
+                    const double lam = line.lambda[p] * factor;
+                    if (lam >= HOLDS_MEAN_LAMBDA) continue;
+                    const double exc = relative_intensity(table, lam, 1.0 - HELD_EXCEEDED);
+                    for (int s = 0; s < STATES; ++s) {
+                        const double m = line.means[p][s] * factor;
+                        if (m > 0.0 && (m * exc >= KURZE_FOLD_HELD_SHARE * held[p][g][s] || !std::isfinite(m * exc))) {
+                            always_held = false;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (always_held) {
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line.energy[p] <= 0.0) continue;
+                    double* target;
+                    if (line.profile >= 0) {
+                        has_prof[line.profile] = true;
+                        target = prof[line.profile][p];
+                    } else {
+                        has_unprof = true;
+                        target = unprof[p];
+                    }
+                    for (int s = 0; s < STATES; ++s)
+                        if (line.means[p][s] > 0.0) target[s] += line.means[p][s];
+                }
+            } else {
+                keep[idx] = true;
+            }
+        }
+        if (!duty_candidates.empty()) {
+            std::sort(duty_candidates.begin(), duty_candidates.end(), [](const auto& a, const auto& b) {
+                return a.first < b.first || (a.first == b.first && a.second < b.second);
+            });
+            double folded_var[PERIODS][STATES] = {};
+            const double budget_sq = DUTY_FOLD_STD_SHARE * DUTY_FOLD_STD_SHARE;
+            for (const auto& candidate : duty_candidates) {
+                const FLine& line = lines[candidate.second];
+                bool fits = true;
+                for (int p = 0; p < PERIODS && fits; ++p) {
+                    if (line.energy[p] <= 0.0 || line.duty[p] <= 0.0) continue;
+                    const double var_scale = std::max(1.0 - line.duty[p], 0.0) / line.duty[p];
+                    for (int s = 0; s < STATES; ++s) {
+                        const double m = line.means[p][s];
+                        if (m <= 0.0) continue;
+                        const double total = layer_total[p][s];
+                        if (!(folded_var[p][s] + m * m * var_scale <= budget_sq * total * total)) {
+                            fits = false;
+                            break;
+                        }
+                    }
+                }
+                if (fits) {
+                    for (int p = 0; p < PERIODS; ++p) {
+                        if (line.energy[p] <= 0.0 || line.duty[p] <= 0.0) continue;
+                        const double var_scale = std::max(1.0 - line.duty[p], 0.0) / line.duty[p];
+                        for (int s = 0; s < STATES; ++s) {
+                            const double m = line.means[p][s];
+                            if (m > 0.0) {
+                                folded_var[p][s] += m * m * var_scale;
+                                unprof[p][s] += m;
+                                has_unprof = true;
+                            }
+                        }
+                    }
+                } else {
+                    keep[candidate.second] = true;
+                }
+            }
+        }
+        std::vector<FLine> folded;
+        if (has_unprof) folded.push_back(FLine::steady(unprof, -1));
+        for (int k = 0; k < PROFILES; ++k)
+            if (has_prof[k]) folded.push_back(FLine::steady(prof[k], k));
+        for (size_t idx = 0; idx < lines.size(); ++idx)
+            if (keep[idx]) folded.push_back(lines[idx]);
+        lines = folded;
+    }
+    merge_close_states(placed, spread);
+    for (FLayer& layer : placed) {
+        if (layer.lines.empty()) continue;
+        const FLine& first = layer.lines.front();
+        bool collapse = first.profile < 0 && !first.has_duty && !first.has_passing;
+        for (int p = 0; p < PERIODS; ++p) collapse = collapse && !std::isfinite(first.lambda[p]);
+        for (int p = 0; p < PERIODS; ++p)
+            for (int s = 0; s < STATES; ++s) collapse = collapse && (float)first.means[p][s] == (float)first.means[p][0];
+        if (collapse) {
+            for (int p = 0; p < PERIODS; ++p) {
+                double sum = 0.0;
+                for (int s = 0; s < STATES; ++s) sum += first.means[p][s];
+                layer.steady[p] += sum / (double)STATES;
+            }
+            layer.lines.erase(layer.lines.begin());
+        }
+    }
+}
+
 }  // namespace loud
 }  // namespace qmo
 
 using namespace qmo::loud;
+
+extern "C" {
+
+// The map's folding of unfolded pixels (the packed layout, every value f64) into the card's packed
+// f32 pixels at the same offsets, and when `out64` is given the same in f64; `spread` the weather
+// merge's (10^0.15 for the map). Returns the floats each folded pixel takes in `used` (at most its
+// unfolded count).
+void qmo_fold_pixels(const double* input, const uint64_t* offsets, size_t pixels, double spread, float* out,
+                     double* out64, uint64_t* used) {
+    for (size_t i = 0; i < pixels; ++i) {
+        const double* in = input + offsets[i];
+        std::vector<FLayer> placed(LAYERS);
+        size_t at = 0;
+        for (int l = 0; l < LAYERS; ++l) {
+            for (int p = 0; p < PERIODS; ++p) placed[l].steady[p] = in[at + p];
+            const int count = (int)in[at + PERIODS];
+            at += PERIODS + 1;
+            for (int k = 0; k < count; ++k) placed[l].lines.push_back(FLine::unpack(in + at + (size_t)k * LINE_FLOATS));
+            at += (size_t)count * LINE_FLOATS;
+        }
+        fold_placed_for_map(placed, spread);
+        float* o = out + offsets[i];
+        double* o64 = out64 ? out64 + offsets[i] : nullptr;
+        size_t w = 0;
+        for (int l = 0; l < LAYERS; ++l) {
+            for (int p = 0; p < PERIODS; ++p) o[w + p] = (float)placed[l].steady[p];
+            o[w + PERIODS] = (float)placed[l].lines.size();
+            if (o64) {
+                for (int p = 0; p < PERIODS; ++p) o64[w + p] = placed[l].steady[p];
+                o64[w + PERIODS] = (double)placed[l].lines.size();
+            }
+            w += PERIODS + 1;
+            for (const FLine& line : placed[l].lines) {
+                line.pack(o + w);
+                if (o64) line.pack(o64 + w);
+                w += LINE_FLOATS;
+            }
+        }
+        for (size_t k = 0; k < (size_t)CURVES * PERIODS * THIRDS; ++k) {
+            o[w + k] = (float)in[at + k];
+            if (o64) o64[w + k] = in[at + k];
+        }
+        used[i] = w + (size_t)CURVES * PERIODS * THIRDS;
+    }
+}
+
+// merge_close_states alone over `layers` layers of packed f64 lines (43 values each, layer l
+// holding counts[l] of them one after another), in place.
+void qmo_merge_close_states(double* lines, const uint32_t* counts, size_t layers, double spread) {
+    std::vector<FLayer> placed(layers);
+    size_t at = 0;
+    for (size_t l = 0; l < layers; ++l) {
+        for (int p = 0; p < PERIODS; ++p) placed[l].steady[p] = 0.0;
+        for (uint32_t k = 0; k < counts[l]; ++k) placed[l].lines.push_back(FLine::unpack(lines + (at + k) * LINE_FLOATS));
+        at += counts[l];
+    }
+    merge_close_states(placed, spread);
+    at = 0;
+    for (size_t l = 0; l < layers; ++l)
+        for (const FLine& line : placed[l].lines) line.pack(lines + (at++) * LINE_FLOATS);
+}
+
+}  // extern "C"
 
 extern "C" {
 

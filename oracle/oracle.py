@@ -102,6 +102,10 @@ def _declare(L):
     L.qmo_line_exceeded_db.restype = C.c_double
     L.qmo_hour_groups.argtypes = [C.POINTER(C.c_int), D]
     L.qmo_pixel_loudness.argtypes = [F32, U64, C.c_size_t, C.c_double, D, D, C.c_int]
+    L.qmo_fold_pixels.argtypes = [D, U64, C.c_size_t, C.c_double, F32, C.c_void_p, U64]
+    L.qmo_fold_pixels.restype = None
+    L.qmo_merge_close_states.argtypes = [D, C.POINTER(C.c_uint32), C.c_size_t, C.c_double]
+    L.qmo_merge_close_states.restype = None
     L.qmo_finite_fraction.argtypes = [C.c_double, C.c_double, C.c_double]
     L.qmo_finite_fraction.restype = C.c_double
     L.qmo_lateral_attenuation.argtypes = [C.c_double, C.c_double]
@@ -491,6 +495,34 @@ def hour_groups():
     factors = np.zeros(36)
     lib().qmo_hour_groups(slots.ctypes.data_as(C.POINTER(C.c_int)), _p(factors, D))
     return slots.reshape(3, 4), factors.reshape(3, 3, 4)
+
+
+MAP_WEATHER_SPREAD = 10.0 ** 0.15
+
+
+def fold_pixels(unfolded, offsets, spread=MAP_WEATHER_SPREAD, f64=False):
+    """The map's folding (paint/src/sone_fold.rs:115-405 fold_placed_for_map, then
+    paint/src/sone.rs:217-234 write_packed) of unfolded pixels (the packed layout in f64) into the
+    card's f32 packed pixels at the same offsets, and the floats each takes; with `f64` also the
+    folded pixels before the cast (NaN past each pixel's end)."""
+    unfolded = np.ascontiguousarray(unfolded, np.float64)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    out = np.full(unfolded.size, np.nan, np.float32)
+    out64 = np.full(unfolded.size, np.nan) if f64 else None
+    used = np.zeros(offsets.size, np.uint64)
+    lib().qmo_fold_pixels(_p(unfolded, D), _p(offsets, U64), offsets.size, float(spread), _p(out, F32),
+                          out64.ctypes.data if f64 else None, _p(used, U64))
+    return (out, used, out64) if f64 else (out, used)
+
+
+def merge_close_states(layers, spread=MAP_WEATHER_SPREAD):
+    """popup/src/percentiles/weather.rs:139-175 merge_close_states over layers of packed f64 lines
+    (a list of (n, 43) arrays); the merged copies."""
+    counts = np.array([len(l) for l in layers], np.uint32)
+    lines = np.ascontiguousarray(np.concatenate([np.asarray(l, np.float64).reshape(-1, 43) for l in layers]))
+    lib().qmo_merge_close_states(_p(lines, D), counts.ctypes.data_as(C.POINTER(C.c_uint32)), counts.size,
+                                 float(spread))
+    return np.split(lines, np.cumsum(counts)[:-1])
 
 
 def pixel_loudness(packed, offsets, bin_db=2.0, mixtures=False, threads=None):

@@ -98,6 +98,8 @@ struct Tables {
     float group_weight[PERIODS][MAX_GROUPS];       // a group's hours / (period hours x STATES)
     double factor[PROFILES][PERIODS][MAX_GROUPS];  // a profile's mean flow factor over a group
     double ln_factorial[SHAPE_CAP + 1];     // ln c! for the passes' Poisson shares
+    double fold_top[ROWS];                  // the map fold's FP64 ln intensity at the last quantile, per row
+    double fold_ln_top, fold_ln_exceed;     // ln of the last quantile's and the held test's probability
 };
 
 // A list of values ascending with the prefix sums before each (LineValues).
@@ -794,6 +796,410 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PIXELS
 }
 
 // =============================================================================================
+// The map's folding (paint/src/sone_fold.rs fold_placed_for_map, popup/src/percentiles/weather.rs
+// merge_close_states, paint/src/sone.rs write_packed): unfolded pixels (the packed layout, every
+// value f64) into the card's packed f32 pixels at the same offsets. A work-item per pixel: the
+// folding is a sequential scan of its lines, cheap beside the distributions; its per-layer sums
+// live in the work-item's private memory, its lines' keep flags and duty ratios in global scratch
+// (one entry per input line), the folded lines in f64 in `work` (the pixel's own range).
+
+constexpr double DUTY_FOLD_PEAK_SHARE = 0.12;
+constexpr double DUTY_FOLD_STD_SHARE = 0.08;
+constexpr double KURZE_FOLD_HELD_SHARE = 0.05;
+constexpr double FOLD_SPREAD = 1.4125375446227544;  // 10^(1.5 / 10)
+constexpr int AT_ENERGY = 0;
+
+__device__ __forceinline__ bool has_duty_of(const double* line) { return !isnan(line[AT_DUTY]); }
+__device__ __forceinline__ bool has_passing_of(const double* line) { return !isnan(line[AT_PASSING]); }
+__device__ __forceinline__ int profile_of(const double* line) { return (int)line[AT_PROFILE]; }
+
+// Kurze's relative intensity at the held test's probability (1 - 1e-3, above the table's last
+// quantile) in FP64, for lambda below the simulated maximum: the top's intensity between the rows
+// about lambda (scaled down with lambda below the table), plus the nearest emitter's passing there
+// less its passing at the top (physics/src/percentile.rs relative_intensity).
+__device__ double exceeded_d(const Tables& T, double lambda) {
+    if (!isfinite(lambda)) return 1.0;
+    if (lambda <= 0.0) return 0.0;
+    const double at = fmax(lambda, LAMBDA_MIN);
+    const double position = fmin(fmax(log10(at / LAMBDA_MIN) * STEPS_PER_DECADE, 0.0), (double)(ROWS - 1));
+    const int row = (int)position;
+    const double fraction = position - trunc(position);
+    const double lo = T.fold_top[row], hi = T.fold_top[min(row + 1, ROWS - 1)];
+    double top = exp(lo + fraction * (hi - lo));
+    if (lambda < LAMBDA_MIN) top = lambda / LAMBDA_MIN * top;
+    auto passing = [&](double ln_p) {
+        const double s = -ln_p / (2.0 * lambda);
+        return 1.0 / (lambda * PI_D * (1.0 + s * s));
+    };
+    return top + passing(T.fold_ln_exceed) - passing(T.fold_ln_top);
+}
+
+// Line::steady into `out`: lambda infinite, no profile unless given, no duty or passes.
+__device__ void write_steady_line(double* out, const double (*means)[STATES], int profile) {
+    for (int p = 0; p < PERIODS; ++p) {
+        bool any = false;
+        for (int k = 0; k < STATES; ++k) {
+            out[AT_MEANS + p * STATES + k] = means[p][k];
+            any = any || means[p][k] > 0.0;
+        }
+        out[AT_ENERGY + p] = any ? 1.0 : 0.0;
+        out[AT_LAMBDA + p] = INFINITY;
+        out[AT_DUTY + p] = NAN;
+        out[AT_PASSING + p] = NAN;
+    }
+    out[AT_PROFILE] = (double)profile;
+}
+
+__global__ __launch_bounds__(64) void fold_kernel(const double* __restrict__ input, const uint64_t* __restrict__ offsets,
+                                                  const uint64_t* __restrict__ line_first, uint32_t pixels,
+                                                  const Tables* __restrict__ tables, double* work, uint8_t* keep,
+                                                  double* ratio, float* out) {
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= pixels) return;
+    const Tables& T = *tables;
+    const double* in = input + offsets[pixel];
+    double* w = work + offsets[pixel];
+    uint8_t* kept = keep + line_first[pixel];
+    double* rat = ratio + line_first[pixel];
+    bool any_profiled = false;
+    {
+        size_t at = 0;
+        for (int l = 0; l < LAYERS; ++l) {
+            const uint32_t n = (uint32_t)in[at + PERIODS];
+            for (uint32_t j = 0; j < n; ++j)
+                any_profiled = any_profiled || profile_of(in + at + PERIODS + 1 + (size_t)j * LINE_FLOATS) >= 0;
+            at += PERIODS + 1 + (size_t)n * LINE_FLOATS;
+        }
+    }
+    size_t at = 0, wat = 0, line_base = 0;
+    size_t layer_w[LAYERS];
+    for (int l = 0; l < LAYERS; ++l) {
+        const double* steady = in + at;
+        const uint32_t n = (uint32_t)in[at + PERIODS];
+        const double* lines = in + at + PERIODS + 1;
+        layer_w[l] = wat;
+        for (int p = 0; p < PERIODS; ++p) w[wat + p] = steady[p];
+        double* count_at = w + wat + PERIODS;
+        wat += PERIODS + 1;
+        if (n == 0) {
+            *count_at = 0.0;
+            at += PERIODS + 1;
+            continue;
+        }
+        bool has_kurze = false, has_duty = false;
+        for (uint32_t j = 0; j < n; ++j) {
+            const double* line = lines + (size_t)j * LINE_FLOATS;
+            if (has_duty_of(line)) has_duty = true;
+            else if (!has_passing_of(line))
+                for (int p = 0; p < PERIODS; ++p) has_kurze = has_kurze || isfinite(line[AT_LAMBDA + p]);
+        }
+        double held[PERIODS][MAX_GROUPS][STATES];
+        if (has_kurze) {
+            for (int p = 0; p < PERIODS; ++p)
+                for (int g = 0; g < MAX_GROUPS; ++g)
+                    for (int k = 0; k < STATES; ++k) held[p][g][k] = steady[p];
+            for (uint32_t j = 0; j < n; ++j) {
+                const double* line = lines + (size_t)j * LINE_FLOATS;
+                if (has_duty_of(line) || has_passing_of(line)) continue;
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line[AT_ENERGY + p] <= 0.0) continue;
+                    const int ng = any_profiled ? T.groups[p] : 1;
+                    for (int g = 0; g < ng; ++g) {
+                        const int profile = profile_of(line);
+                        const double factor = any_profiled && profile >= 0 ? T.factor[profile][p][g] : 1.0;
+                        for (int k = 0; k < STATES; ++k) {
+                            const double m = line[AT_MEANS + p * STATES + k] * factor;
+                            if (m > 0.0) held[p][g][k] += m;
+                        }
+                    }
+                }
+            }
+        }
+        double total[PERIODS][STATES];
+        if (has_duty) {
+            for (int p = 0; p < PERIODS; ++p)
+                for (int k = 0; k < STATES; ++k) total[p][k] = steady[p];
+            for (uint32_t j = 0; j < n; ++j) {
+                const double* line = lines + (size_t)j * LINE_FLOATS;
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line[AT_ENERGY + p] <= 0.0) continue;
+                    for (int k = 0; k < STATES; ++k)
+                        if (line[AT_MEANS + p * STATES + k] > 0.0) total[p][k] += line[AT_MEANS + p * STATES + k];
+                }
+            }
+        }
+        double unprof[PERIODS][STATES], prof[PROFILES][PERIODS][STATES];
+        for (int p = 0; p < PERIODS; ++p)
+            for (int k = 0; k < STATES; ++k) {
+                unprof[p][k] = 0.0;
+                for (int q = 0; q < PROFILES; ++q) prof[q][p][k] = 0.0;
+            }
+        bool has_unprof = false, has_prof[PROFILES] = {false, false, false};
+        uint32_t candidates = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const double* line = lines + (size_t)j * LINE_FLOATS;
+            kept[line_base + j] = 0;
+            rat[line_base + j] = NAN;
+            if (has_duty_of(line)) {
+                bool moves = false, eligible = true;
+                double max_ratio = 0.0;
+                for (int p = 0; p < PERIODS; ++p) {
+                    const double d = line[AT_DUTY + p];
+                    if (line[AT_ENERGY + p] <= 0.0 || d <= 0.0) continue;
+                    const double var_scale = fmax(1.0 - d, 0.0) / d;
+                    for (int k = 0; k < STATES; ++k) {
+                        const double mean = line[AT_MEANS + p * STATES + k];
+                        if (mean <= 0.0) continue;
+                        moves = true;
+                        const double t = total[p][k];
+                        if (t <= 0.0 || mean / d > DUTY_FOLD_PEAK_SHARE * t) {
+                            eligible = false;
+                        } else {
+                            const double r = mean / t;
+                            max_ratio = fmax(max_ratio, r * r * var_scale);
+                        }
+                    }
+                }
+                if (moves) {
+                    if (eligible) {
+                        rat[line_base + j] = max_ratio;
+                        ++candidates;
+                    } else {
+                        kept[line_base + j] = 1;
+                    }
+                }
+                continue;
+            }
+            if (has_passing_of(line)) {
+                bool moves = false;
+                for (int p = 0; p < PERIODS; ++p) {
+                    bool any = false;
+                    for (int k = 0; k < STATES; ++k) any = any || line[AT_MEANS + p * STATES + k] > 0.0;
+                    const double passing = line[AT_PASSING + p];
+                    moves = moves || (line[AT_ENERGY + p] > 0.0 && passing > 0.0 && passing < PASSES_HOLD_MEAN && any);
+                }
+                if (moves) {
+                    kept[line_base + j] = 1;
+                } else {
+                    for (int p = 0; p < PERIODS; ++p) {
+                        if (line[AT_ENERGY + p] <= 0.0) continue;
+                        for (int k = 0; k < STATES; ++k)
+                            if (line[AT_MEANS + p * STATES + k] > 0.0) {
+                                unprof[p][k] += line[AT_MEANS + p * STATES + k];
+                                has_unprof = true;
+                            }
+                    }
+                }
+                continue;
+            }
+            const int profile = profile_of(line);
+            bool always_held = true;
+            for (int p = 0; p < PERIODS && always_held; ++p) {
+                if (line[AT_ENERGY + p] <= 0.0) continue;
+                const int ng = any_profiled ? T.groups[p] : 1;
+                for (int g = 0; g < ng && always_held; ++g) {
+                    const double factor = any_profiled && profile >= 0 ? T.factor[profile][p][g] : 1.0;
+                    const double lambda = line[AT_LAMBDA + p] * factor;
+                    if (lambda >= HOLDS_MEAN_LAMBDA) continue;
+                    const double exc = exceeded_d(T, lambda);
+                    for (int k = 0; k < STATES; ++k) {
+                        const double m = line[AT_MEANS + p * STATES + k] * factor;
+                        if (m > 0.0 && (m * exc >= KURZE_FOLD_HELD_SHARE * held[p][g][k] || !isfinite(m * exc))) {
+                            always_held = false;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (always_held) {
+                for (int p = 0; p < PERIODS; ++p) {
+                    if (line[AT_ENERGY + p] <= 0.0) continue;
+                    double* target;
+                    if (profile >= 0) {
+                        has_prof[profile] = true;
+                        target = prof[profile][p];
+                    } else {
+                        has_unprof = true;
+                        target = unprof[p];
+                    }
+                    for (int k = 0; k < STATES; ++k)
+                        if (line[AT_MEANS + p * STATES + k] > 0.0) target[k] += line[AT_MEANS + p * STATES + k];
+                }
+            } else {
+                kept[line_base + j] = 1;
+            }
+        }
+        // The events eligible to fold, in order of their variance ratio (then their index), while
+        // the folded variance stays within the budget.
+        if (candidates > 0) {
+            double folded_var[PERIODS][STATES];
+            for (int p = 0; p < PERIODS; ++p)
+                for (int k = 0; k < STATES; ++k) folded_var[p][k] = 0.0;
+            const double budget_sq = DUTY_FOLD_STD_SHARE * DUTY_FOLD_STD_SHARE;
+            double last_ratio = -1.0;
+            int64_t last_index = -1;
+            for (uint32_t c = 0; c < candidates; ++c) {
+                int64_t pick = -1;
+                double pick_ratio = 0.0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const double r = rat[line_base + j];
+                    if (isnan(r)) continue;
+                    const bool after = r > last_ratio || (r == last_ratio && (int64_t)j > last_index);
+                    const bool before = pick < 0 || r < pick_ratio || (r == pick_ratio && (int64_t)j < pick);
+                    if (after && before) {
+                        pick = j;
+                        pick_ratio = r;
+                    }
+                }
+                last_ratio = pick_ratio;
+                last_index = pick;
+                const double* line = lines + (size_t)pick * LINE_FLOATS;
+                bool fits = true;
+                for (int p = 0; p < PERIODS && fits; ++p) {
+                    const double d = line[AT_DUTY + p];
+                    if (line[AT_ENERGY + p] <= 0.0 || d <= 0.0) continue;
+                    const double var_scale = fmax(1.0 - d, 0.0) / d;
+                    for (int k = 0; k < STATES; ++k) {
+                        const double m = line[AT_MEANS + p * STATES + k];
+                        if (m <= 0.0) continue;
+                        if (!(folded_var[p][k] + m * m * var_scale <= budget_sq * total[p][k] * total[p][k])) {
+                            fits = false;
+                            break;
+                        }
+                    }
+                }
+                if (fits) {
+                    for (int p = 0; p < PERIODS; ++p) {
+                        const double d = line[AT_DUTY + p];
+                        if (line[AT_ENERGY + p] <= 0.0 || d <= 0.0) continue;
+                        const double var_scale = fmax(1.0 - d, 0.0) / d;
+                        for (int k = 0; k < STATES; ++k) {
+                            const double m = line[AT_MEANS + p * STATES + k];
+                            if (m > 0.0) {
+                                folded_var[p][k] += m * m * var_scale;
+                                unprof[p][k] += m;
+                                has_unprof = true;
+                            }
+                        }
+                    }
+                } else {
+                    kept[line_base + pick] = 1;
+                }
+            }
+        }
+        // The folded steady lines first (unprofiled, then by profile), then the kept lines in order.
+        uint32_t count = 0;
+        if (has_unprof) {
+            write_steady_line(w + wat, unprof, -1);
+            wat += LINE_FLOATS;
+            ++count;
+        }
+        for (int q = 0; q < PROFILES; ++q)
+            if (has_prof[q]) {
+                write_steady_line(w + wat, prof[q], q);
+                wat += LINE_FLOATS;
+                ++count;
+            }
+        for (uint32_t j = 0; j < n; ++j)
+            if (kept[line_base + j]) {
+                const double* line = lines + (size_t)j * LINE_FLOATS;
+                for (int k = 0; k < LINE_FLOATS; ++k) w[wat + k] = line[k];
+                wat += LINE_FLOATS;
+                ++count;
+            }
+        *count_at = (double)count;
+        line_base += n;
+        at += PERIODS + 1 + (size_t)n * LINE_FLOATS;
+    }
+    const size_t curves_in = at;
+    // merge_close_states over every line of the pixel, period by period.
+    for (int p = 0; p < PERIODS; ++p) {
+        int start = 0;
+        while (start < STATES) {
+            int end = start + 1;
+            for (; end < STATES; ++end) {
+                bool all = true;
+                for (int l = 0; l < LAYERS && all; ++l) {
+                    const size_t base = layer_w[l];
+                    const uint32_t n = (uint32_t)w[base + PERIODS];
+                    for (uint32_t j = 0; j < n && all; ++j) {
+                        const double* m = w + base + PERIODS + 1 + (size_t)j * LINE_FLOATS + AT_MEANS + p * STATES;
+                        double low = INFINITY, high = 0.0;
+                        for (int k = start; k <= end; ++k) {
+                            if (!isfinite(m[k])) {
+                                all = false;
+                                break;
+                            }
+                            low = fmin(low, m[k]);
+                            high = fmax(high, m[k]);
+                        }
+                        all = all && high <= FOLD_SPREAD * low;
+                    }
+                }
+                if (!all) break;
+            }
+            if (end - start > 1)
+                for (int l = 0; l < LAYERS; ++l) {
+                    const size_t base = layer_w[l];
+                    const uint32_t n = (uint32_t)w[base + PERIODS];
+                    for (uint32_t j = 0; j < n; ++j) {
+                        double* m = w + base + PERIODS + 1 + (size_t)j * LINE_FLOATS + AT_MEANS + p * STATES;
+                        bool differs = false;
+                        for (int k = start + 1; k < end; ++k) differs = differs || m[k] != m[start];
+                        if (!differs) continue;
+                        double sum = 0.0;
+                        for (int k = start; k < end; ++k) sum += m[k];
+                        const double mean = sum / (double)(end - start);
+                        for (int k = start; k < end; ++k) m[k] = mean;
+                    }
+                }
+            start = end;
+        }
+    }
+    // Each layer's first line, if an unprofiled steady line whose states all read alike in FP32,
+    // goes back into the layer's steady floor; then the packed f32 pixel.
+    float* o = out + offsets[pixel];
+    size_t oat = 0;
+    for (int l = 0; l < LAYERS; ++l) {
+        const size_t base = layer_w[l];
+        double steady[PERIODS] = {w[base], w[base + 1], w[base + 2]};
+        uint32_t n = (uint32_t)w[base + PERIODS];
+        const double* first = w + base + PERIODS + 1;
+        uint32_t skip = 0;
+        if (n > 0 && profile_of(first) < 0 && !has_duty_of(first) && !has_passing_of(first)) {
+            bool collapse = true;
+            for (int p = 0; p < PERIODS; ++p) {
+                collapse = collapse && !isfinite(first[AT_LAMBDA + p]);
+                for (int k = 0; k < STATES; ++k)
+                    collapse = collapse && (float)first[AT_MEANS + p * STATES + k] == (float)first[AT_MEANS + p * STATES];
+            }
+            if (collapse) {
+                for (int p = 0; p < PERIODS; ++p) {
+                    double sum = 0.0;
+                    for (int k = 0; k < STATES; ++k) sum += first[AT_MEANS + p * STATES + k];
+                    steady[p] += sum / (double)STATES;
+                }
+                skip = 1;
+            }
+        }
+        for (int p = 0; p < PERIODS; ++p) o[oat + p] = (float)steady[p];
+        o[oat + PERIODS] = (float)(n - skip);
+        oat += PERIODS + 1;
+        for (uint32_t j = skip; j < n; ++j) {
+            const double* line = w + base + PERIODS + 1 + (size_t)j * LINE_FLOATS;
+            for (int k = 0; k < LINE_FLOATS; ++k) o[oat + k] = (float)line[k];
+            // An event's duty and a passing line's passes go whole or not at all (the first tells).
+            for (int p = 0; p < PERIODS; ++p) {
+                if (!has_duty_of(line)) o[oat + AT_DUTY + p] = NAN;
+                if (!has_passing_of(line)) o[oat + AT_PASSING + p] = NAN;
+            }
+            oat += LINE_FLOATS;
+        }
+    }
+    for (int k = 0; k < CURVES * PERIODS * THIRDS; ++k) o[oat + k] = (float)in[curves_in + k];
+}
+
+// =============================================================================================
 // Host side.
 
 int pfail(char* err, size_t errlen, const char* fmt, ...) {
@@ -930,6 +1336,8 @@ void build_tables(const float* quantiles, Tables& T) {
     T.top_ln_p = (float)std::log(top);
     for (int c = 0; c <= SHAPE_CAP; ++c) T.ln_factorial[c] = std::lgamma((double)c + 1.0);
     T.exceed_ln_p = (float)std::log(1.0 - HELD_EXCEEDED);
+    T.fold_ln_exceed = std::log(1.0 - HELD_EXCEEDED);
+    T.fold_ln_top = std::log(top);
     auto log_row = [&](double prob, float* out) {
         prob = std::max(prob, quantile_probability(0));
         const int k = std::min((int)std::floor(prob * QUANTILES - 0.5), QUANTILES - 2);
@@ -940,6 +1348,7 @@ void build_tables(const float* quantiles, Tables& T) {
             const double here = std::log(std::max((double)q[k], 1e-30));
             const double next = std::log(std::max((double)q[k + 1], 1e-30));
             out[row] = (float)(here + t * (next - here));
+            if (prob == top) T.fold_top[row] = here + t * (next - here);
         }
     };
     for (int n = 0; n < T.first_tail; ++n) log_row(p[n], T.log_rows[n]);
@@ -959,9 +1368,11 @@ void build_tables(const float* quantiles, Tables& T) {
         for (int g = 0; g < runs; ++g) {
             const int first = g * hours / runs, last = (g + 1) * hours / runs;
             T.group_weight[period][g] = (float)((double)(last - first) / ((double)hours * STATES));
+            std::vector<int> group(slots.begin() + first, slots.begin() + last);
+            std::sort(group.begin(), group.end());
             for (int profile = 0; profile < PROFILES; ++profile) {
                 double f = 0.0;
-                for (int s = first; s < last; ++s) f += hour_factor(profile, period, slots[s]);
+                for (int s : group) f += hour_factor(profile, period, s);
                 T.factor[profile][period][g] = f / (double)(last - first);
             }
         }
@@ -970,23 +1381,24 @@ void build_tables(const float* quantiles, Tables& T) {
 
 // The packed pixels' shape, checked on the host: each layer's line count a whole number the
 // input holds, each line's profile none or a known one, the curves within the input.
-int validate(const float* input, size_t input_floats, const uint64_t* offsets, size_t pixels, char* err,
+template <typename V>
+int validate(const V* input, size_t input_floats, const uint64_t* offsets, size_t pixels, char* err,
              size_t errlen) {
     for (size_t i = 0; i < pixels; ++i) {
         size_t at = offsets[i];
         for (int l = 0; l < LAYERS; ++l) {
             if (at > input_floats || input_floats - at < (size_t)PERIODS + 1)
                 return pfail(err, errlen, "pixel %zu: layer %d beyond the input", i, l);
-            const float count = input[at + PERIODS];
-            if (!(count >= 0.0f && count <= (float)LINES_MAX) || count != std::floor(count))
+            const double count = (double)input[at + PERIODS];
+            if (!(count >= 0.0 && count <= (double)LINES_MAX) || count != std::floor(count))
                 return pfail(err, errlen, "pixel %zu: layer %d line count %g", i, l, (double)count);
             const size_t lines = (size_t)count;
             at += PERIODS + 1;
             if (input_floats - at < lines * LINE_FLOATS)
                 return pfail(err, errlen, "pixel %zu: layer %d lines beyond the input", i, l);
             for (size_t j = 0; j < lines; ++j) {
-                const float profile = input[at + j * LINE_FLOATS + AT_PROFILE];
-                if (!(profile == -1.0f || profile == 0.0f || profile == 1.0f || profile == 2.0f))
+                const double profile = (double)input[at + j * LINE_FLOATS + AT_PROFILE];
+                if (!(profile == -1.0 || profile == 0.0 || profile == 1.0 || profile == 2.0))
                     return pfail(err, errlen, "pixel %zu: layer %d line %zu profile %g", i, l, j, (double)profile);
             }
             at += lines * LINE_FLOATS;
@@ -1044,5 +1456,70 @@ extern "C" int qm_pixel_loudness(qm_gpu* gpu, const float* input, size_t input_f
     if (e == hipSuccess && mixtures)
         e = hipMemcpy(mixtures, d_mix.p, sizeof(float) * mix_floats, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return pfail(err, errlen, "pixel loudness: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int qm_pixel_loudness_unfolded(qm_gpu* gpu, const double* unfolded, size_t input_floats,
+                                          const uint64_t* offsets, size_t pixels, const float* kurze,
+                                          float* mixtures, float* ndens, float* folded, char* err, size_t errlen) {
+    if (!gpu || (pixels && (!unfolded || !offsets || !ndens))) return pfail(err, errlen, "null argument");
+    if (pixels == 0) return 0;
+    if (pixels > (size_t)INT32_MAX) return pfail(err, errlen, "%zu pixels exceed one launch", pixels);
+    if (validate(unfolded, input_floats, offsets, pixels, err, errlen)) return 1;
+    // Each pixel's first line among all the input's lines: the fold's per-line flags live there.
+    std::vector<uint64_t> line_first(pixels);
+    uint64_t lines = 0;
+    for (size_t i = 0; i < pixels; ++i) {
+        line_first[i] = lines;
+        size_t at = offsets[i];
+        for (int l = 0; l < LAYERS; ++l) {
+            const uint64_t n = (uint64_t)unfolded[at + PERIODS];
+            lines += n;
+            at += PERIODS + 1 + n * LINE_FLOATS;
+        }
+    }
+    Tables* tables = new (std::nothrow) Tables;
+    if (!tables) return pfail(err, errlen, "out of host memory");
+    build_tables(kurze ? kurze : kurze_quantiles().data(), *tables);
+    const size_t mix_floats = pixels * PERIODS * CURVES * BINS * 2;
+    Device d_input, d_offsets, d_first, d_tables, d_work, d_keep, d_ratio, d_out, d_mix, d_ndens;
+    hipError_t e = hipSetDevice(gpu->ordinal);
+    if (e == hipSuccess) e = hipMalloc(&d_input.p, sizeof(double) * std::max<size_t>(input_floats, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_work.p, sizeof(double) * std::max<size_t>(input_floats, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_out.p, sizeof(float) * std::max<size_t>(input_floats, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_offsets.p, sizeof(uint64_t) * pixels);
+    if (e == hipSuccess) e = hipMalloc(&d_first.p, sizeof(uint64_t) * pixels);
+    if (e == hipSuccess) e = hipMalloc(&d_keep.p, std::max<uint64_t>(lines, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_ratio.p, sizeof(double) * std::max<uint64_t>(lines, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_tables.p, sizeof(Tables));
+    if (e == hipSuccess && mixtures) e = hipMalloc(&d_mix.p, sizeof(float) * mix_floats);
+    if (e == hipSuccess) e = hipMalloc(&d_ndens.p, sizeof(float) * pixels * CURVES);
+    if (e == hipSuccess) e = hipMemcpy(d_input.p, unfolded, sizeof(double) * input_floats, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_offsets.p, offsets, sizeof(uint64_t) * pixels, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_first.p, line_first.data(), sizeof(uint64_t) * pixels, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_tables.p, tables, sizeof(Tables), hipMemcpyHostToDevice);
+    delete tables;
+    // Floats past a folded pixel's end read NaN (all bits set).
+    if (e == hipSuccess) e = hipMemset(d_out.p, 0xFF, sizeof(float) * input_floats);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((pixels + 63) / 64)), dim3(64), 0, 0,
+                           (const double*)d_input.p, (const uint64_t*)d_offsets.p, (const uint64_t*)d_first.p,
+                           (uint32_t)pixels, (const Tables*)d_tables.p, (double*)d_work.p, (uint8_t*)d_keep.p,
+                           (double*)d_ratio.p, (float*)d_out.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(pixel_loudness_kernel, dim3((unsigned)pixels), dim3(GROUP), 0, 0, (const float*)d_out.p,
+                           (const uint64_t*)d_offsets.p, (const Tables*)d_tables.p, (float*)d_mix.p,
+                           (float*)d_ndens.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(ndens, d_ndens.p, sizeof(float) * pixels * CURVES, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mixtures)
+        e = hipMemcpy(mixtures, d_mix.p, sizeof(float) * mix_floats, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && folded)
+        e = hipMemcpy(folded, d_out.p, sizeof(float) * input_floats, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return pfail(err, errlen, "pixel loudness of unfolded pixels: %s", hipGetErrorString(e));
     return 0;
 }

@@ -39,6 +39,7 @@ EXPORTS = (
     "qm_crossings",
     "qm_evaluate_line_pairs",
     "qm_pixel_loudness",
+    "qm_pixel_loudness_unfolded",
     "qm_kurze_table",
     "qm_horizons",
     "qm_flight_boxes_upload",
@@ -102,6 +103,7 @@ def library():
         lib.qm_aircraft_grid.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, u32, u32, dbl, dbl, vp, vp, vp, vp, vp, vp,
                                          cp, sz]
         lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
+        lib.qm_pixel_loudness_unfolded.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_kurze_table.argtypes = [vp]
         lib.qm_kurze_table.restype = None
         lib.qm_horizons.argtypes = [vp, vp, vp, vp, sz, vp, vp, cp, sz]
@@ -346,6 +348,26 @@ class Gpu:
                                          _ptr(mix) if mixtures else None, _ptr(nd), err, 512)
         _check(rc, err)
         return (nd, mix) if mixtures else nd
+
+    def pixel_loudness_unfolded(self, unfolded, offsets, kurze=None, mixtures=False, folded=False):
+        """Nden per curve (pixels, 7) of unfolded pixels (the packed layout in f64, lines as placed),
+        folded for the map on the card first; with `mixtures` the per-period mixtures and with
+        `folded` the folded f32 pixels (NaN past each pixel's end) too, in that order."""
+        unfolded = np.ascontiguousarray(unfolded, np.float64)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        n = offsets.size
+        nd = np.zeros((n, 7), np.float32)
+        mix = np.zeros((n, 3, 7, 90, 2), np.float32) if mixtures else None
+        out = np.zeros(unfolded.size, np.float32) if folded else None
+        table = None if kurze is None else np.ascontiguousarray(kurze, np.float32)
+        err = C.create_string_buffer(512)
+        rc = library().qm_pixel_loudness_unfolded(self.handle, _ptr(unfolded), unfolded.size, _ptr(offsets), n,
+                                                  _ptr(table) if table is not None else None,
+                                                  _ptr(mix) if mixtures else None, _ptr(nd),
+                                                  _ptr(out) if folded else None, err, 512)
+        _check(rc, err)
+        result = (nd,) + ((mix,) if mixtures else ()) + ((out,) if folded else ())
+        return result[0] if len(result) == 1 else result
 
     def horizons(self, square: Square, points, altitude_m=None):
         """Terrain and building horizons of `points` (Horizons::build, write_f32): floats

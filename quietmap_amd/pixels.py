@@ -115,3 +115,90 @@ def synthetic_pixels(seed=7, count=64, lines_max=6, kurze=True):
                 curves[c, :, 0] = np.nan
         pixels.append(pack_pixel(layers, curves))
     return pixels
+
+
+def pack_line_f64(energy, means, lam, profile=None, duty=None, passing=None):
+    """One unfolded line's 43 values in f64 (the packed layout before the map's folding)."""
+    out = np.empty(LINE_FLOATS, np.float64)
+    out[0:3] = energy
+    out[3:33] = np.asarray(means, np.float64).reshape(-1)
+    out[33:36] = lam
+    out[36] = -1.0 if profile is None else float(profile)
+    out[37:40] = np.nan if duty is None else duty
+    out[40:43] = np.nan if passing is None else passing
+    return out
+
+
+def pack_pixel_f64(layers, curves):
+    """One unfolded pixel (f64): as pack_pixel, the lines as placed (pack_line_f64)."""
+    assert len(layers) == LAYERS
+    parts = []
+    for steady, lines in layers:
+        parts.append(np.asarray(steady, np.float64).reshape(PERIODS))
+        parts.append(np.array([len(lines)], np.float64))
+        parts.extend(np.asarray(line, np.float64) for line in lines)
+    parts.append(np.asarray(curves, np.float64).reshape(-1))
+    return np.concatenate(parts)
+
+
+def concat_f64(pixels):
+    """Unfolded pixels one after another (f64) and their offsets (values)."""
+    offsets = np.zeros(len(pixels), np.uint64)
+    at = 0
+    for i, p in enumerate(pixels):
+        offsets[i] = at
+        at += p.size
+    return np.concatenate(pixels).astype(np.float64), offsets
+
+
+def synthetic_unfolded(seed=7, count=64, lines_max=8):
+    """Seeded unfolded pixels (f64) with every kind of line the map's folding sorts: weak and loud
+    events on a duty, sparse Kurze lines (held, moving, following hourly profiles), steady lines
+    whose weather states differ by less than the merge's spread, streams of passes that hold and
+    that move. Zero energy in a period goes with zero means there. Returns (values, offsets)."""
+    rng = np.random.default_rng(seed)
+    pixels = []
+    for _ in range(count):
+        parts = []
+        any_profile = rng.uniform() < 0.5
+        for layer in range(LAYERS):
+            if rng.uniform() < 0.25:
+                parts.append(np.array([0.0, 0.0, 0.0, 0.0]))
+                continue
+            level = rng.uniform(30, 70)
+            steady = 10 ** ((level - rng.uniform(-5, 20)) / 10) * np.array([1.0, 0.6, 0.3])
+            lines = []
+            for _ in range(rng.integers(0, lines_max + 1)):
+                e = 10 ** ((level + rng.normal(-6, 8)) / 10) * np.array([1.0, rng.uniform(0.3, 1), rng.uniform(0.05, 1)])
+                if rng.uniform() < 0.15:
+                    e[rng.integers(0, 3)] = 0.0
+                if rng.uniform() < 0.4:
+                    spread = 10 ** (rng.uniform(-0.05, 0.05, (PERIODS, STATES)))  # within the merge spread
+                else:
+                    spread = 10 ** (rng.normal(0, 0.25, (PERIODS, STATES)))
+                means = e[:, None] * spread / spread.mean(axis=1, keepdims=True)
+                if rng.uniform() < 0.3:
+                    means[:, 5:] = means[:, 4:5]
+                kind = rng.uniform()
+                if kind < 0.3:
+                    lam = 10 ** rng.uniform(-5.5, 1.7, 3)
+                    profile = int(rng.integers(0, 3)) if any_profile and rng.uniform() < 0.6 else None
+                    lines.append(pack_line_f64(e, means, lam, profile=profile))
+                elif kind < 0.45:
+                    lines.append(pack_line_f64(e, means, [np.inf] * 3))
+                elif kind < 0.7:
+                    duty = rng.uniform(0.001, 0.6, 3)
+                    duty[rng.uniform(size=3) < 0.2] = 0.0
+                    lines.append(pack_line_f64(e * rng.uniform(0.001, 1.0), means * rng.uniform(0.001, 0.05), [np.inf] * 3,
+                                               duty=duty))
+                else:
+                    passing = 10 ** rng.uniform(-4, 2.2, 3)
+                    lines.append(pack_line_f64(e, np.repeat(e[:, None], STATES, 1), [np.inf] * 3, passing=passing))
+            parts.append(np.concatenate([steady, [float(len(lines))]] + lines))
+        curves = np.stack([np.stack([_curve(rng) for _ in range(PERIODS)]) for _ in range(CURVES)])
+        for c in range(CURVES):
+            if rng.uniform() < 0.1:
+                curves[c, :, 0] = np.nan
+        parts.append(curves.reshape(-1))
+        pixels.append(np.concatenate(parts))
+    return concat_f64(pixels)
