@@ -73,9 +73,12 @@ __global__ __launch_bounds__(256) void line_nodes_kernel(Scene sc, const qm_rece
 
 template <bool FINE>
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(2))) void line_rays_kernel(
-    Scene sc, BandTable B, const qm_receiver* receivers, const qm_line_piece* pieces, const uint32_t* pair_point,
+    const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
+    const qm_line_piece* pieces, const uint32_t* pair_point,
     const uint32_t* pair_piece, const uint32_t* ray_pair, const uint32_t* ray_first, uint32_t n_rays,
     const LineNode* nodes, const float* divergence, float* sums, uint32_t* failed, float* pair_bands) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_rays) return;
     PathScratch w;
@@ -326,11 +329,11 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
     if (total) {
         if (square->scene.terrain.fine_detail)
             hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
-                               square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
+                               square->d_scene, square->d_bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
                                (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
         else
             hipLaunchKernelGGL(line_rays_kernel<false>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
-                               square->scene, gpu->bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
+                               square->d_scene, square->d_bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
                                (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);
         L_TRY(hipGetLastError());
     }

@@ -72,11 +72,13 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 // FINE: the square holds fine terrain (its own instantiation, so the coarse kernel keeps its
 // registers).
 template <bool FINE>
-__global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(Scene sc, BandTable B, const qm_receiver* receivers,
+__global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
                                                     uint32_t n, float* sums, uint32_t* failed, float* point_bands,
                                                     float* pair_bands) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathScratch w;
@@ -107,7 +109,9 @@ __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(Q
 }
 
 template <bool FINE>
-__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(Scene sc, BandTable B, SourceGrid sg, GridArgs g) {
+__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, SourceGrid sg, GridArgs g) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     __shared__ uint32_t s_tile;
     PathScratch w;
     const int lane = threadIdx.x;
@@ -324,10 +328,10 @@ int launch_paint(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_rece
     groups = std::min<unsigned>(groups, g.n_tiles);
     groups = std::max<unsigned>(groups, 1);
     if (fine)
-        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->d_scene, square->d_bands,
                            square->grid, g);
     else
-        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->scene, gpu->bands,
+        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->d_scene, square->d_bands,
                            square->grid, g);
     QM_TRY(hipGetLastError());
     return 0;
@@ -497,6 +501,14 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     sq->grid.bucket_m = bucket_m;
     sq->grid.buckets_x = bx;
     sq->grid.buckets_y = by;
+    // The scene's and the band table's records in device memory: the path kernels read them
+    // through pointers (scalar loads where used) rather than holding them in their arguments.
+    qm::Scene* d_scene;
+    qm::BandTable* d_bands;
+    if (upload(sq, &sq->scene, 1, &d_scene, err, errlen)) return cleanup(1);
+    if (upload(sq, &gpu->bands, 1, &d_bands, err, errlen)) return cleanup(1);
+    sq->d_scene = d_scene;
+    sq->d_bands = d_bands;
     *out = sq;
     return 0;
 }
@@ -567,13 +579,13 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
         return 1;
     const unsigned blocks = (unsigned)((n_pairs + 255) / 256);
     if (square->scene.terrain.fine_detail)
-        hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
+        hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
                            (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
                            (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
                            (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
                            (float*)(pair_bands ? d_pair_bands.p : nullptr));
     else
-        hipLaunchKernelGGL(pairs_kernel<false>, dim3(blocks), dim3(256), 0, stream, square->scene, gpu->bands,
+        hipLaunchKernelGGL(pairs_kernel<false>, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
                            (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
                            (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
                            (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),

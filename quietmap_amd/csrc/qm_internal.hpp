@@ -26,6 +26,8 @@ struct SourceGrid {
 struct qm_square {
     qm_gpu* gpu;
     qm::Scene scene;  // device pointers
+    const qm::Scene* d_scene;        // `scene` itself in device memory
+    const qm::BandTable* d_bands;    // the GPU's band table in device memory
     SourceGrid grid;  // device pointers
     size_t n_sources;
     qm_source* d_sources_by_input;  // input order (for pairs)

@@ -952,12 +952,7 @@ struct PathResult {
 
 // ---------------------------------------------------------------------------------------------
 // One state's boundary term (diffraction.rs:66-196), FP64 deltas, FP32 bands.
-#ifdef QM_BOUNDARY_NOINLINE
-__device__ __noinline__
-#else
-QM_DEV
-#endif
-void state_boundary(const BandTable& B, StateStream& st, Hull& hull, const CandLog& log, bool favourable, P2f Sf, P2f Rf, double z0,
+QM_DEV void state_boundary(const BandTable& B, StateStream& st, Hull& hull, const CandLog& log, bool favourable, P2f Sf, P2f Rf, double z0,
                            double L, double pz_L, double pxz_L, double pg_L, const Geometry& direct,
                            float path_ground, float near_source, float source_g, float g_start,
                            uint32_t& failed, float att[BANDS]) {

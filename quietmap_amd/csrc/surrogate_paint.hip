@@ -53,8 +53,11 @@ __device__ BucketBox bucket_box(const SourceGrid& sg, double x, double y, double
 }  // namespace
 
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(2))) void samples_kernel(
-    Scene sc, BandTable B, const qm_receiver* receivers, const qm_source* sources, const uint32_t* pair_receiver,
-    const uint32_t* pair_source, uint32_t n, float* inputs, float* targets, uint32_t* in_envelope) {
+    const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
+    const qm_source* sources, const uint32_t* pair_receiver, const uint32_t* pair_source, uint32_t n, float* inputs,
+    float* targets, uint32_t* in_envelope) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathScratch w;
@@ -190,8 +193,11 @@ __global__ __launch_bounds__(256) void features_kernel(Scene sc, TileGrid g, con
 }
 
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(2))) void exact_kernel(
-    Scene sc, BandTable B, TileGrid g, const qm_source* sources, const uint32_t* item_tile, const uint32_t* item_source,
-    const uint32_t* exact_list, const uint32_t* exact_count, float* exact_energy, uint8_t* exact_failed) {
+    const Scene* __restrict__ scene, const BandTable* __restrict__ bands, TileGrid g, const qm_source* sources,
+    const uint32_t* item_tile, const uint32_t* item_source, const uint32_t* exact_list, const uint32_t* exact_count,
+    float* exact_energy, uint8_t* exact_failed) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     const uint32_t count = *exact_count;
     PathScratch w;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
@@ -358,8 +364,8 @@ extern "C" int qm_surrogate_samples_device(qm_gpu* gpu, const qm_square* square,
     if (square->scene.terrain.fine_detail)
         return fail(err, errlen, "fine terrain: the network is trained on the coarse walk");
     SP_TRY(hipSetDevice(gpu->ordinal));
-    hipLaunchKernelGGL(samples_kernel, dim3(blocks_for(n_pairs)), dim3(QM_PATH_GROUP), 0, (hipStream_t)stream, square->scene,
-                       gpu->bands, d_receivers, square->d_sources_by_input, d_pair_receiver, d_pair_source,
+    hipLaunchKernelGGL(samples_kernel, dim3(blocks_for(n_pairs)), dim3(QM_PATH_GROUP), 0, (hipStream_t)stream, square->d_scene,
+                       square->d_bands, d_receivers, square->d_sources_by_input, d_pair_receiver, d_pair_source,
                        (uint32_t)n_pairs, d_inputs, d_targets, d_in_envelope);
     SP_TRY(hipGetLastError());
     return 0;
@@ -472,8 +478,8 @@ extern "C" int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square*
                                d_row_of, d_flags, d_source_alt, d_exact_list, d_exact_count);
             SP_TRY(hipGetLastError());
             if (qm::mlp_infer_rows(mlp, d_rows, items * 256 * 2, d_row_count, d_att, s, err, errlen)) return 1;
-            hipLaunchKernelGGL(exact_kernel, dim3((unsigned)std::max(cus, 1) * 4), dim3(QM_PATH_GROUP), 0, s, square->scene,
-                               gpu->bands, g, square->grid.sources, d_item_tile + item0, d_item_source + item0,
+            hipLaunchKernelGGL(exact_kernel, dim3((unsigned)std::max(cus, 1) * 4), dim3(QM_PATH_GROUP), 0, s, square->d_scene,
+                               square->d_bands, g, square->grid.sources, d_item_tile + item0, d_item_source + item0,
                                d_exact_list, d_exact_count, d_exact_energy, d_exact_failed);
             SP_TRY(hipGetLastError());
         }

@@ -49,8 +49,10 @@ struct TilesArgs {
 
 }  // namespace
 
-__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void tiles_kernel(Scene sc, BandTable B,
-                                                                                                TilesArgs g) {
+__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void tiles_kernel(
+    const Scene* __restrict__ scene, const BandTable* __restrict__ bands, TilesArgs g) {
+    const Scene& sc = *scene;
+    const BandTable& B = *bands;
     __shared__ uint32_t s_item;
     PathScratch w;
     const int lane = threadIdx.x;
@@ -206,7 +208,7 @@ int build(qm_tile_plan* p, const qm_square* square, const qm_tile* tiles, size_t
     hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(256), 0, p->stream, d_paths, n_receivers);
     hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(256), 0, p->stream, d_failures, n_receivers);
     hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counters, (size_t)XCDS);
-    hipLaunchKernelGGL(tiles_kernel, dim3(groups), dim3(GROUP), 0, p->stream, square->scene, p->gpu->bands, g);
+    hipLaunchKernelGGL(tiles_kernel, dim3(groups), dim3(GROUP), 0, p->stream, square->d_scene, square->d_bands, g);
     const hipError_t launched = hipGetLastError();
     const hipError_t ended = hipStreamEndCapture(p->stream, &p->graph);
     T_TRY(launched);
