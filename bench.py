@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-loudness", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-flights", action="store_true")
+    ap.add_argument("--no-doc29", action="store_true")
+    ap.add_argument("--doc29-scale", type=float, default=1.0, help="side of the Doc 29 grid relative to 2048")
+    ap.add_argument("--doc29-bands", type=int, default=2, help="launches (bands of rows) the Doc 29 grid takes")
     ap.add_argument("--loudness-cells", type=int, default=16 * 1024 * 1024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -231,6 +234,72 @@ def main():
         handle_f.close()
         print(f"[bench] flights {fl['value']:.3e} box reads/s", file=sys.stderr, flush=True)
 
+    # The Doc 29 leg: the aircraft configuration's 2048 x 2048 grid at 50 m (20,000 tracks of 200
+    # segments, 100 classes, 16 km reach) in bands of rows, one aircraft_kernel launch a band; Lden,
+    # N65/N70 and the 0.5 dB LAmax histogram per receiver, checked on sampled receivers.
+    d29 = None
+    if not args.no_doc29:
+        from quietmap_amd import scenes as scenes_a
+
+        ac = scenes_a.aircraft_scene(scale=args.doc29_scale)
+        nxa, nya = ac.grid_shape
+        na = nxa * nya
+        energy_a = np.zeros((na, 3))
+        n65_a = np.zeros(na, np.uint32)
+        n70_a = np.zeros(na, np.uint32)
+        hist_a = np.zeros((na, abi.HIST_BINS), np.uint32)
+        bands = max(1, min(args.doc29_bands, nya))
+        edges = [nya * k // bands for k in range(bands + 1)]
+        t_a = time.time()
+        for k in range(bands):
+            r0, r1 = edges[k], edges[k + 1]
+            sub = scenes_a.AircraftScene(ac.points_xy[r0 * nxa:r1 * nxa], (nxa, r1 - r0), ac.classes, ac.tracks,
+                                         ac.vertices, ac.receiver_altitude_m, ac.reach_m)
+            tb = time.time()
+            r = gpu.aircraft_grid(sub)
+            sl = slice(r0 * nxa, r1 * nxa)
+            energy_a[sl], n65_a[sl], n70_a[sl], hist_a[sl] = r["period_energy"], r["n65"], r["n70"], r["hist"]
+            print(f"[bench] doc29 band {k + 1}/{bands}: rows {r0}-{r1}, {time.time() - tb:.2f} s", file=sys.stderr,
+                  flush=True)
+        wall_a = time.time() - t_a
+        d29 = {"workload": f"Doc 29 grid {nxa}x{nya} at 50 m, {ac.tracks.size} tracks of 200 segments, "
+                           f"{ac.classes.size} classes, {ac.reach_m:.0f} m reach",
+               "receivers": na, "launches": bands, "wall_s": wall_a, "receivers_per_s": na / wall_a,
+               "kernel": "aircraft_kernel", "timing": "host wall of the launches with their transfers"}
+        if not args.no_cpu:
+            from oracle import oracle
+
+            oracle.use_native()
+            threads = args.cpu_threads or min(os.cpu_count() or 1, CPU_SHARE)
+            pick = np.random.default_rng(29).choice(na, 4 * threads, replace=False)
+            want_a = oracle.aircraft_points(ac, ac.points_xy[pick], threads=threads)
+            cpu_a = pick.size / median_seconds(lambda: oracle.aircraft_points(ac, ac.points_xy[pick], threads=threads))
+            one_a = 8 / median_seconds(lambda: oracle.aircraft_points(ac, ac.points_xy[pick[:8]], threads=1))
+            heard = float(want_a["segments"].mean())
+            lden_g, lden_w = lden_db(energy_a[pick]), lden_db(want_a["period_energy"])
+            ok = np.isfinite(lden_w)
+            # About 160 FP32 operations a heard segment (closest point, NPD read, duration, lateral
+            # and installation terms, energy), transcendentals counted as one.
+            flop = 160.0 * heard * na / wall_a / 1e12
+            d29.update({
+                "segments_heard_per_receiver": heard,
+                "segment_evaluations_per_s": heard * na / wall_a,
+                "roofline": {"bound": "valu", "achieved": flop, "peak": 157.3, "unit": "TFLOP/s",
+                             "frac": flop / 157.3, "traffic": None, "kernel": "aircraft_kernel",
+                             "flop_per_heard_segment": 160},
+                "parity_vs_fp64_oracle": {"receivers": int(pick.size),
+                                          "max_abs_lden_db": float(np.abs(lden_g[ok] - lden_w[ok]).max()),
+                                          "silent_equal": bool(np.array_equal(np.isfinite(lden_g), ok)),
+                                          "n65_n70_equal": bool(np.array_equal(n65_a[pick], want_a["n65"]) and
+                                                                np.array_equal(n70_a[pick], want_a["n70"])),
+                                          "hist_equal": bool(np.array_equal(hist_a[pick], want_a["hist"]))},
+                "cpu_baseline": {"value": cpu_a, "unit": "receivers/s", "cores": threads, "kind": "port",
+                                 "one_thread": one_a, **host_cpu(), "median_of": 3, "build": oracle.NATIVE_FLAGS,
+                                 "sample": f"{pick.size} random receivers of the grid (8 on one thread), FP64 C++ "
+                                           "restatement of the reference's CPU path"},
+            })
+        print(f"[bench] doc29 {na / wall_a:.3e} receivers/s over {bands} launches", file=sys.stderr, flush=True)
+
     # The loudness leg: ISO 532-1 on band-major planes of 28 third-octave levels per cell.
     loud = None
     if not args.no_loudness:
@@ -373,6 +442,7 @@ def main():
         "loudness": loud,
         "surrogate": sur,
         "flights": fl,
+        "doc29": d29,
         "gpu": gpu.name,
     }
     print(json.dumps(line), flush=True)
