@@ -206,6 +206,28 @@ int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, const qm_point*
                            const uint32_t* pair_piece, size_t n_pairs, float* sums, uint32_t* failed, uint32_t* rays,
                            float* point_bands, float* pair_bands, char* err, size_t errlen);
 
+/* The square's line candidates (DeviceSquare::upload keeps point and line candidates alike,
+ * paint-gpu/src/square_upload.rs:172-220; popup/src/candidates.rs:89-104 Candidate with `line`):
+ * line piece k is candidate n_sources + k of qm_evaluate_candidates. Replaces the lines set before;
+ * ends must be finite and layers below QM_LAYERS. */
+int qm_square_set_lines(qm_square* square, const qm_line_piece* pieces, size_t n_pieces, char* err, size_t errlen);
+/* The square's candidates: its point sources, then its line pieces. */
+size_t qm_square_candidates(const qm_square* square);
+
+/* Gpu::evaluate_pairs as GpuBatch calls it (paint-gpu/src/device.rs:434-765, its argument
+ * checks at 443-471): per pair its candidate (a point source or a line piece of the square) and
+ * its receiver among `receivers`; per receiver its point in `bands` (`point_of_receiver`, null:
+ * receiver r is point r). Out: each pair's 9 sums and failure bits (qm_evaluate_pairs' layout),
+ * each pair's own mixed bands (`pair_spectra`, optional, n_pairs*3*8 floats, unweighted), and
+ * `bands` (optional, n_points*6*3*8 floats) ADDED to: every successful pair's bands times its
+ * `band_weight` (null: 1) at its receiver's point, layer by the candidate's (exact.rs:208-216).
+ * A candidate outside the square, a receiver out of range or a point outside the bands fails the
+ * call before any work. Host arrays. */
+int qm_evaluate_candidates(qm_gpu* gpu, const qm_square* square, const qm_point* receivers, size_t n_receivers,
+                           const uint32_t* point_of_receiver, size_t n_points, const uint32_t* candidate,
+                           const uint32_t* receiver_of_pair, const float* band_weight, size_t n_pairs, float* sums,
+                           uint32_t* failed, float* bands, float* pair_spectra, char* err, size_t errlen);
+
 /* The crossing list of each segment (from, to) in frame metres, as the exact painter finds it
  * (popup/src/obstacles/crossings.rs:163-230; parity hook): per segment up to `capacity` crossings
  * (t, footprint, outline, edge index, flags bit 0 building / bit 1 bounds roof) and the count. */

@@ -4,9 +4,10 @@ at each point of a pass, on the card.
 
 - ``evaluate(points, pairs)``: every pair's energy (mixed per period, calm and downwind alone) and
   every point's bands per layer, period and band, the pairs' band weights applied to the bands
-  (exact.rs:208-216 add_bands). Pairs go to the card in order of candidate (batch.rs:73-90
-  by_candidate), at most ``PAIRS_PER_LAUNCH`` a launch, each launch's points once
-  (batch.rs:111-160 LaunchInputs::gather). A pair the card refuses (failure bits) is listed in
+  (exact.rs:208-216 add_bands). Candidates are the square's point sources and line pieces alike.
+  Pairs go to the card in order of candidate (batch.rs:73-90 by_candidate), at most
+  ``PAIRS_PER_LAUNCH`` a launch, each launch's points once (batch.rs:111-160
+  LaunchInputs::gather), the bands summed on the card at each reached point's slot. A pair the card refuses (failure bits) is listed in
   ``Evaluated.refused`` with its bits: the reference evaluates those on its CPU path
   (batch.rs:292-317), which stays the caller's.
 - ``pair_spectra(points, pairs)``: each pair's unweighted energy and bands (hum_pairs.rs).
@@ -62,23 +63,27 @@ class Evaluated:
 
 
 class GpuBatch:
-    """batch.rs GpuBatch over one uploaded square (``painter.Square``) and its candidates."""
+    """batch.rs GpuBatch over one uploaded square (``painter.Square``) and its candidates: its
+    point sources, then its line pieces (``Square.set_lines``)."""
 
     def __init__(self, gpu, square, pairs_per_launch=PAIRS_PER_LAUNCH):
         self.gpu, self.square = gpu, square
-        self.candidates = int(square.sources.size)
+        self.candidates = int(square.candidates)
         self.pairs_per_launch = int(pairs_per_launch)
 
-    def _launches(self, points, pairs, point_bands):
-        """Yields (launch pair indices, the card's results) for the pairs in candidate order."""
-        points = np.ascontiguousarray(points, abi.POINT_DTYPE)
-        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
-        if pairs.size and int(pairs["point"].max()) >= points.size:
-            raise PainterError("pair point out of range")
+    def _launches(self, points, pairs, bands):
+        """Yields (launch pair indices, the card's results) for the pairs in candidate order, each
+        launch's receivers its points once in the order its pairs first reach them
+        (LaunchInputs::gather) and each receiver's bands added at its point's slot in `bands`
+        (batch.rs:251-261: every point a pair reaches, once, in that order)."""
         order = by_candidate(pairs, self.candidates)
+        slot_of = np.full(points.size, np.iinfo(np.uint32).max, np.uint32)
+        if bands is not None:
+            unique, first = np.unique(pairs["point"], return_index=True)
+            reached = unique[np.argsort(first, kind="stable")]
+            slot_of[reached] = np.arange(reached.size, dtype=np.uint32)
         for at in range(0, order.size, self.pairs_per_launch):
             launch = order[at : at + self.pairs_per_launch]
-            # LaunchInputs::gather: each point once, in the order its pairs first come.
             pts = pairs["point"][launch]
             unique, first = np.unique(pts, return_index=True)
             receivers = unique[np.argsort(first, kind="stable")]
@@ -86,25 +91,36 @@ class GpuBatch:
             slot[receivers] = np.arange(receivers.size, dtype=np.uint32)
             w = pairs["band_weight"][launch]
             weights = None if np.all(w == 1.0) else w.astype(np.float32)
-            out = self.gpu.evaluate_pairs(self.square, points[receivers], slot[pts], pairs["candidate"][launch],
-                                          pair_weight=weights, point_bands=point_bands, pair_bands=not point_bands)
-            yield launch, receivers, out
+            out = self.gpu.evaluate_candidates(self.square, points[receivers], pairs["candidate"][launch], slot[pts],
+                                               point_of_receiver=slot_of[receivers] if bands is not None else None,
+                                               band_weight=weights, bands=bands, pair_spectra=bands is None)
+            yield launch, out
+
+    def _checked(self, points, pairs):
+        points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        if pairs.size and int(pairs["point"].max()) >= points.size:
+            raise PainterError("pair point out of range")
+        return points, pairs
 
     def evaluate(self, points, pairs) -> Evaluated:
-        n_points = np.size(points)
-        n = np.size(pairs)
+        points, pairs = self._checked(points, pairs)
+        n = pairs.size
         mixed, states = np.zeros((n, abi.PERIODS)), np.zeros((n, 2, abi.PERIODS))
-        bands = np.zeros((n_points, abi.LAYERS, abi.PERIODS, abi.BANDS))
+        unique, first = np.unique(pairs["point"], return_index=True)
+        reached = unique[np.argsort(first, kind="stable")]
+        flat = np.zeros((reached.size, abi.LAYERS, abi.PERIODS, abi.BANDS), np.float32)
         refused, bits = [], []
-        for launch, receivers, out in self._launches(points, pairs, point_bands=True):
+        for launch, out in self._launches(points, pairs, flat):
             sums = out["sums"].astype(np.float64)
             mixed[launch] = sums[:, 0:3]
             states[launch, 0] = sums[:, 3:6]
             states[launch, 1] = sums[:, 6:9]
-            bands[receivers] += out["point_bands"].astype(np.float64)
             bad = out["failed"] != 0
             refused.append(launch[bad].astype(np.int64))
             bits.append(out["failed"][bad])
+        bands = np.zeros((points.size, abi.LAYERS, abi.PERIODS, abi.BANDS))
+        bands[reached] = flat.astype(np.float64)
         refused = np.concatenate(refused) if refused else np.zeros(0, np.int64)
         bits = np.concatenate(bits) if bits else np.zeros(0, np.uint32)
         order = np.argsort(refused, kind="stable")
@@ -113,16 +129,17 @@ class GpuBatch:
     def pair_spectra(self, points, pairs):
         """Each pair's unweighted energy (mixed[3], states[2][3]) and its bands [3][8]; refused
         pairs (failure bits) zero, listed as in ``evaluate``."""
-        n = np.size(pairs)
+        points, pairs = self._checked(points, pairs)
+        n = pairs.size
         mixed, states = np.zeros((n, abi.PERIODS)), np.zeros((n, 2, abi.PERIODS))
         spectra = np.zeros((n, abi.PERIODS, abi.BANDS))
         refused, bits = [], []
-        for launch, _, out in self._launches(points, pairs, point_bands=False):
+        for launch, out in self._launches(points, pairs, None):
             sums = out["sums"].astype(np.float64)
             mixed[launch] = sums[:, 0:3]
             states[launch, 0] = sums[:, 3:6]
             states[launch, 1] = sums[:, 6:9]
-            spectra[launch] = out["pair_bands"].astype(np.float64)
+            spectra[launch] = out["pair_spectra"].astype(np.float64)
             bad = out["failed"] != 0
             refused.append(launch[bad].astype(np.int64))
             bits.append(out["failed"][bad])

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "libqmpaint.so")
-SOURCES = [os.path.join(CSRC, f) for f in ("painter.hip", "loudness.hip", "aircraft.hip", "surrogate.hip", "surrogate_paint.hip", "tiles.hip", "lines.hip", "pixels.hip", "horizons.hip", "flights.hip", "raster.hip")]
+SOURCES = [os.path.join(CSRC, f) for f in ("painter.hip", "loudness.hip", "aircraft.hip", "surrogate.hip", "surrogate_paint.hip", "tiles.hip", "lines.hip", "pixels.hip", "horizons.hip", "flights.hip", "raster.hip", "candidates.hip")]
 HEADERS = [os.path.join(CSRC, f) for f in ("qm_scene.h", "qm_aircraft.h", "qm_device.hpp", "qm_path.hpp", "qm_eval.hpp", "qm_queue.hpp", "qm_internal.hpp", "qm_features.hpp", "qm_line.hpp", "qm_zwicker.hpp", "qm_doc29.hpp")] + [
     os.path.join(os.path.dirname(HERE), "include", "qm_paint.h"),
     os.path.join(os.path.dirname(HERE), "include", "qm_surrogate.h"),

@@ -31,6 +31,7 @@ struct qm_square {
     SourceGrid grid;  // device pointers
     size_t n_sources;
     qm_source* d_sources_by_input;  // input order (for pairs)
+    std::vector<qm_line_piece> lines;  // line candidates, numbered after the point sources
     std::vector<void*> allocations;
     size_t bytes;
 };

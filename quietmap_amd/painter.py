@@ -40,6 +40,9 @@ EXPORTS = (
     "qm_evaluate_line_pairs",
     "qm_pixel_loudness",
     "qm_pixel_loudness_unfolded",
+    "qm_square_set_lines",
+    "qm_square_candidates",
+    "qm_evaluate_candidates",
     "qm_kurze_table",
     "qm_horizons",
     "qm_flight_boxes_upload",
@@ -104,6 +107,10 @@ def library():
                                          cp, sz]
         lib.qm_pixel_loudness.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, cp, sz]
         lib.qm_pixel_loudness_unfolded.argtypes = [vp, vp, sz, vp, sz, vp, vp, vp, vp, cp, sz]
+        lib.qm_square_set_lines.argtypes = [vp, vp, sz, cp, sz]
+        lib.qm_square_candidates.argtypes = [vp]
+        lib.qm_square_candidates.restype = sz
+        lib.qm_evaluate_candidates.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp, vp, sz, vp, vp, vp, vp, cp, sz]
         lib.qm_kurze_table.argtypes = [vp]
         lib.qm_kurze_table.restype = None
         lib.qm_horizons.argtypes = [vp, vp, vp, vp, sz, vp, vp, cp, sz]
@@ -153,10 +160,29 @@ class Square:
 
     def __init__(self, gpu, handle, scene, sources):
         self.gpu, self.handle, self.scene, self.sources = gpu, handle, scene, sources
+        self.lines = np.zeros(0, abi.LINE_PIECE_DTYPE)
 
     @property
     def bytes(self):
         return library().qm_square_bytes(self.handle)
+
+    def set_lines(self, pieces):
+        """The square's line candidates (numbered after its point sources)."""
+        pieces = np.ascontiguousarray(pieces, abi.LINE_PIECE_DTYPE)
+        err = C.create_string_buffer(512)
+        _check(library().qm_square_set_lines(self.handle, _ptr(pieces), pieces.size, err, 512), err)
+        self.lines = pieces
+
+    @property
+    def candidates(self):
+        """Point sources, then line pieces."""
+        return int(library().qm_square_candidates(self.handle))
+
+    def layer_of(self, candidate):
+        """The layer of each candidate index."""
+        c = np.asarray(candidate, np.int64)
+        layers = np.concatenate([self.sources["layer"], self.lines["layer"]]).astype(np.int64)
+        return layers[c]
 
     def close(self):
         if self.handle:
@@ -348,6 +374,37 @@ class Gpu:
                                          _ptr(mix) if mixtures else None, _ptr(nd), err, 512)
         _check(rc, err)
         return (nd, mix) if mixtures else nd
+
+    def evaluate_candidates(self, square: Square, receivers, candidate, receiver_of_pair, point_of_receiver=None,
+                            n_points=None, band_weight=None, bands=None, pair_spectra=False):
+        """Gpu::evaluate_pairs as GpuBatch calls it: pairs of point and line candidates of one
+        square, each pair's receiver among `receivers`, each receiver's point in `bands` (an
+        array [n_points, 6, 3, 8] float32 added to; None: no bands). Returns a dict of `sums`
+        [n, 9], `failed` [n] and, with `pair_spectra`, `pair_spectra` [n, 3, 8]."""
+        rcv = np.ascontiguousarray(receivers, abi.POINT_DTYPE)
+        cand = np.ascontiguousarray(candidate, np.uint32)
+        rop = np.ascontiguousarray(receiver_of_pair, np.uint32)
+        n = cand.size
+        assert rop.size == n
+        por = None if point_of_receiver is None else np.ascontiguousarray(point_of_receiver, np.uint32)
+        assert por is None or por.size == rcv.size
+        w = None if band_weight is None else np.ascontiguousarray(band_weight, np.float32)
+        assert w is None or w.size == n
+        if bands is not None:
+            assert bands.dtype == np.float32 and bands.flags["C_CONTIGUOUS"]
+            n_points = bands.size // (abi.LAYERS * abi.PERIODS * abi.BANDS)
+        sums = np.zeros((n, abi.PAIR_SUMS), np.float32)
+        failed = np.zeros(n, np.uint32)
+        spectra = np.zeros((n, abi.PERIODS, abi.BANDS), np.float32) if pair_spectra else None
+        err = C.create_string_buffer(512)
+        rc = library().qm_evaluate_candidates(self.handle, square.handle, _ptr(rcv), rcv.size, _ptr(por),
+                                              int(n_points or 0), _ptr(cand), _ptr(rop), _ptr(w), n, _ptr(sums),
+                                              _ptr(failed), _ptr(bands), _ptr(spectra), err, 512)
+        _check(rc, err)
+        out = dict(sums=sums, failed=failed)
+        if pair_spectra:
+            out["pair_spectra"] = spectra
+        return out
 
     def pixel_loudness_unfolded(self, unfolded, offsets, kurze=None, mixtures=False, folded=False):
         """Nden per curve (pixels, 7) of unfolded pixels (the packed layout in f64, lines as placed),

@@ -1,6 +1,7 @@
 """The batch's host logic (quietmap_amd/batch.py, the reference's GpuBatch, paint-gpu/src/batch.rs)
 over a stand-in card that answers each pair with a known function of its point and candidate:
-pairs launched in candidate order at most `pairs_per_launch` at a time, each launch's points once,
+pairs of point and line candidates launched in candidate order at most `pairs_per_launch` at a time,
+each launch's points once and each reached point's slot in the band sums,
 energies back in the pairs' own order, weighted bands summed per point and layer across launches,
 refused pairs listed with their bits; too many pixels in a loudness batch and indices out of range
 refused. The card itself is checked in tests/test_gpu_batch.py."""
@@ -13,56 +14,65 @@ from quietmap_amd.painter import PainterError
 
 
 class FakeSquare:
-    def __init__(self, n):
+    """Candidates: n point sources, then `lines` line pieces (layers cycling over both)."""
+
+    def __init__(self, n, lines=0):
         self.sources = np.zeros(n, abi.SOURCE_DTYPE)
         self.sources["layer"] = np.arange(n) % abi.LAYERS
+        self.lines = np.zeros(lines, abi.LINE_PIECE_DTYPE)
+        self.lines["layer"] = (np.arange(lines) + 3) % abi.LAYERS
+        self.candidates = n + lines
+
+    def layer_of(self, candidate):
+        return np.concatenate([self.sources["layer"], self.lines["layer"]])[np.asarray(candidate, np.int64)]
 
 
 class FakeGpu:
-    """Answers pair (point at x, candidate c): mixed = x + c per period (+ period), calm 2x,
-    downwind 3x; bands all in band 0; refuses candidates divisible by 17."""
+    """Answers pair (receiver at x, candidate c): mixed = x + c per period (+ period), calm 2x,
+    downwind 3x; bands all in band 0, added at the receiver's point with the pair's weight;
+    refuses candidates divisible by 17."""
 
     def __init__(self, square):
         self.square = square
         self.launches = []
 
-    def evaluate_pairs(self, square, points, pair_point, pair_source, pair_weight=None, point_bands=True,
-                       pair_bands=False):
-        self.launches.append((points.copy(), pair_point.copy(), pair_source.copy()))
-        assert np.unique(points["x_m"]).size == points.size  # each point once
-        assert np.all(np.diff(pair_source.astype(np.int64)) >= 0)  # candidate order
-        n = pair_point.size
-        base = points["x_m"][pair_point] + pair_source
+    def evaluate_candidates(self, square, receivers, candidate, receiver_of_pair, point_of_receiver=None,
+                            n_points=None, band_weight=None, bands=None, pair_spectra=False):
+        self.launches.append((receivers.copy(), receiver_of_pair.copy(), candidate.copy()))
+        assert np.unique(receivers["x_m"]).size == receivers.size  # each point once
+        assert np.all(np.diff(candidate.astype(np.int64)) >= 0)  # candidate order
+        assert candidate.max() < square.candidates
+        n = receiver_of_pair.size
+        base = receivers["x_m"][receiver_of_pair] + candidate
         sums = np.zeros((n, 9), np.float32)
         for q in range(3):
             sums[:, q] = base + q
             sums[:, 3 + q] = 2 * (base + q)
             sums[:, 6 + q] = 3 * (base + q)
-        failed = np.where(pair_source % 17 == 0, 4, 0).astype(np.uint32)
+        failed = np.where(candidate % 17 == 0, 4, 0).astype(np.uint32)
         sums[failed != 0] = 0
-        out = dict(sums=sums, failed=failed, point_bands=None, pair_bands=None)
-        w = np.ones(n) if pair_weight is None else pair_weight
-        if point_bands:
-            pb = np.zeros((points.size, abi.LAYERS, 3, 8), np.float32)
+        out = dict(sums=sums, failed=failed)
+        w = np.ones(n) if band_weight is None else band_weight
+        if bands is not None:
+            por = np.arange(receivers.size) if point_of_receiver is None else point_of_receiver
             ok = failed == 0
             for q in range(3):
-                np.add.at(pb[:, :, q, 0], (pair_point[ok], square.sources["layer"][pair_source[ok]]),
+                np.add.at(bands[:, :, q, 0], (por[receiver_of_pair[ok]], square.layer_of(candidate[ok])),
                           (w[ok] * sums[ok, q]).astype(np.float32))
-            out["point_bands"] = pb
-        if pair_bands:
+        if pair_spectra:
             sp = np.zeros((n, 3, 8), np.float32)
             sp[:, :, 0] = sums[:, 0:3]
-            out["pair_bands"] = sp
+            out["pair_spectra"] = sp
         return out
 
 
-def setup(n_points=50, n_candidates=60, n_pairs=700, seed=1):
+def setup(n_points=50, n_candidates=60, n_pairs=700, seed=1, lines=20):
     rng = np.random.default_rng(seed)
     points = np.zeros(n_points, abi.POINT_DTYPE)
     points["x_m"] = np.arange(n_points) * 1000.0
-    pairs = pairs_of(rng.integers(0, n_points, n_pairs), rng.integers(0, n_candidates, n_pairs),
+    pairs = pairs_of(rng.integers(0, n_points, n_pairs), rng.integers(0, n_candidates + lines, n_pairs),
                      np.where(rng.uniform(size=n_pairs) < 0.3, rng.uniform(0.1, 5.0, n_pairs), 1.0))
-    square = FakeSquare(n_candidates)
+    square = FakeSquare(n_candidates, lines)
     return points, pairs, square
 
 
@@ -82,7 +92,7 @@ def test_evaluate_orders_gathers_and_sums(per_launch):
     assert np.array_equal(out.refused, np.flatnonzero(refused)) and np.all(out.refused_bits == 4)
     bands = np.zeros((points.size, abi.LAYERS, 3))
     for k in np.flatnonzero(~refused):
-        layer = square.sources["layer"][pairs["candidate"][k]]
+        layer = square.layer_of(pairs["candidate"][k])
         bands[pairs["point"][k], layer] += pairs["band_weight"][k] * (base[k] + np.arange(3))
     np.testing.assert_allclose(out.bands[:, :, :, 0], bands, rtol=1e-5)
     assert not out.bands[:, :, :, 1:].any()
