@@ -192,17 +192,18 @@ void skyline_arcs(const qm_obstacles& O, const double origin[2], double lo_rad, 
     steps_of(O, origin[0], origin[1], steps);
     const double ox = steps[0], oy = steps[1];
     const double mps = O.metres_per_step;
+    const double mpy = O.metres_per_step_y > 0.0 ? O.metres_per_step_y : mps;
     const double cell = QM_CELL_STEPS;
     const double low[2] = {std::cos(lo_rad), std::sin(lo_rad)}, high[2] = {std::cos(hi_rad), std::sin(hi_rad)};
     auto relative = [&](int32_t vx, int32_t vy, double out[2]) {
         out[0] = ((double)vx - ox) * mps;
-        out[1] = (oy - (double)vy) * mps;
+        out[1] = (oy - (double)vy) * mpy;
     };
-    const long row0 = (long)std::floor((oy - radius_m / mps) / cell), row1 = (long)std::floor((oy + radius_m / mps) / cell);
+    const long row0 = (long)std::floor((oy - radius_m / mpy) / cell), row1 = (long)std::floor((oy + radius_m / mpy) / cell);
     const long col0 = (long)std::floor((ox - radius_m / mps) / cell), col1 = (long)std::floor((ox + radius_m / mps) / cell);
     for (long row = row0; row <= row1; ++row) {
-        const double south = (oy - (row + 1) * cell) * mps;
-        const double north = (oy - row * cell) * mps;
+        const double south = (oy - (row + 1) * cell) * mpy;
+        const double north = (oy - row * cell) * mpy;
         const double dy = std::max(std::max(south, -north), 0.0);
         for (long column = col0; column <= col1; ++column) {
             const double west = (column * cell - ox) * mps;

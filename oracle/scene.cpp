@@ -88,10 +88,51 @@ static double detail_at(const qm_terrain& T, double row, double column) {
     return sum * QM_DETAIL_STEP_M;
 }
 
+// The geographic frame (qm_terrain.row_y_m, column_m): the one arc-second rows at the table's
+// north metres, fine rows evenly between them (popup/src/scene.rs:303-330 row_line_y,
+// rows_north_of), the columns evenly. Without a table, row r at origin_y + r * spacing.
+static double column_metres(const qm_terrain& T) { return T.column_m > 0.0 ? T.column_m : T.spacing_m; }
+static double table_row_y(const qm_terrain& T, int64_t k) {
+    const int64_t last = (int64_t)T.rows - 1;
+    if (k < 0) return T.row_y_m[0] + (double)k * (T.row_y_m[1] - T.row_y_m[0]);
+    if (k > last) return T.row_y_m[last] + (double)(k - last) * (T.row_y_m[last] - T.row_y_m[last - 1]);
+    return T.row_y_m[k];
+}
+// North metres of row line k of the lattice `scale` times as fine (a table's rows only).
+static double row_line_y(const qm_terrain& T, int scale, int64_t k) {
+    const int64_t coarse = k >= 0 ? k / scale : -((-k + scale - 1) / scale);
+    const int64_t sub = k - coarse * scale;
+    const double y0 = table_row_y(T, coarse);
+    if (sub == 0) return y0;
+    return y0 + (table_row_y(T, coarse + 1) - y0) * (double)sub / (double)scale;
+}
+static double column_position(const qm_terrain& T, double x) {
+    return T.column_m > 0.0 ? (x - T.origin_x_m) / T.column_m : (x - T.origin_x_m) / T.spacing_m;
+}
+// The fractional row (node index) of north metres y: the table's rows about it.
+static double row_position(const qm_terrain& T, double y) {
+    if (!T.row_y_m) return (y - T.origin_y_m) / T.spacing_m;
+    const int64_t last = (int64_t)T.rows - 1;
+    int64_t lo = 0;
+    if (y < T.row_y_m[0]) {
+        lo = 0;
+    } else if (y >= T.row_y_m[last]) {
+        lo = last - 1;
+    } else {
+        int64_t hi = last;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (T.row_y_m[mid] <= y) lo = mid;
+            else hi = mid;
+        }
+    }
+    return (double)lo + (y - T.row_y_m[lo]) / (T.row_y_m[lo + 1] - T.row_y_m[lo]);
+}
+
 // popup/src/scene.rs Ground::at: the 1/6" height where the point's block has fine terrain.
 int ground_at(const qm_terrain& T, double x, double y, GroundSample& out) {
-    const double cp = (x - T.origin_x_m) / T.spacing_m;
-    const double rp = (y - T.origin_y_m) / T.spacing_m;
+    const double cp = column_position(T, x);
+    const double rp = row_position(T, y);
     const int failed = cell_sample(T, cp, rp, out);
     if (failed) return failed;
     if (fine_block(T, cp, rp)) out.height_m += detail_at(T, QM_FINE_PER_NODE * rp, QM_FINE_PER_NODE * cp);
@@ -167,10 +208,15 @@ static void push_sample(Profile& profile, double t, const GroundSample& s) {
 static int walk(const qm_terrain& T, int scale, const double a[2], const double b[2], double from, double to,
                 Profile& profile) {
     const double unit = T.spacing_m / scale;
-    const double ca = (a[0] - T.origin_x_m) / unit, cb = (b[0] - T.origin_x_m) / unit;
-    const double ra = (a[1] - T.origin_y_m) / unit, rb = (b[1] - T.origin_y_m) / unit;
+    const double cunit = column_metres(T) / scale;
+    const double ca = (a[0] - T.origin_x_m) / cunit, cb = (b[0] - T.origin_x_m) / cunit;
+    // With a row table the row lines lie at their north metres (rise in metres, ra 0) and the row
+    // index of the start comes from the table (scene.rs walk: rows_north_of).
+    const bool geo = T.row_y_m != nullptr;
+    const double ra = geo ? 0.0 : (a[1] - T.origin_y_m) / unit, rb = geo ? b[1] - a[1] : (b[1] - T.origin_y_m) / unit;
     const double columns = cb - ca, rise = rb - ra;
-    const double column_from = ca + from * columns, row_from = ra + from * rise;
+    const double column_from = ca + from * columns;
+    const double row_from = geo ? scale * row_position(T, a[1] + from * rise) : ra + from * rise;
     double next_column, column_step;
     if (columns > 0.0) {
         next_column = std::floor(column_from) + 1.0;
@@ -195,7 +241,16 @@ static int walk(const qm_terrain& T, int scale, const double a[2], const double 
     const double inf = std::numeric_limits<double>::infinity();
     for (;;) {
         const double t_column = columns != 0.0 ? (next_column - ca) / columns : inf;
-        const double t_row = rise != 0.0 ? (next_row - ra) / rise : inf;
+        const double t_row = rise == 0.0 ? inf
+                             : geo      ? (row_line_y(T, scale, (int64_t)next_row) - a[1]) / rise
+                                        : (next_row - ra) / rise;
+        // A column vertex's fraction between the row lines about it.
+        auto row_fraction = [&](double t) {
+            if (!geo) return std::min(std::max(ra + t * rise - south, 0.0), 1.0);
+            const double y = a[1] + t * rise, y0 = row_line_y(T, scale, (int64_t)south);
+            const double y1 = row_line_y(T, scale, (int64_t)south + 1);
+            return std::min(std::max((y - y0) / (y1 - y0), 0.0), 1.0);
+        };
         const double t = std::min(t_column, t_row);
         if (t >= to) break;
         GroundSample sample;
@@ -211,8 +266,7 @@ static int walk(const qm_terrain& T, int scale, const double a[2], const double 
                 h0 = node_at(T, (int64_t)next_row, (int64_t)west, n0);
                 h1 = node_at(T, (int64_t)next_row, (int64_t)west + 1, n1);
             } else {
-                const double r = ra + t * rise;
-                f = std::min(std::max(r - south, 0.0), 1.0);
+                f = row_fraction(t);
                 h0 = node_at(T, (int64_t)south, (int64_t)next_column, n0);
                 h1 = node_at(T, (int64_t)south + 1, (int64_t)next_column, n1);
             }
@@ -221,9 +275,7 @@ static int walk(const qm_terrain& T, int scale, const double a[2], const double 
             const double c = ca + t * columns;
             failed = fine_sample(T, true, (int64_t)next_row, 0, c, c - std::floor(c), sample);
         } else {
-            const double r = ra + t * rise;
-            failed = fine_sample(T, false, (int64_t)next_column, (int64_t)south, r,
-                                 std::min(std::max(r - south, 0.0), 1.0), sample);
+            failed = fine_sample(T, false, (int64_t)next_column, (int64_t)south, 0.0, row_fraction(t), sample);
         }
         if (failed) return failed;
         push_sample(profile, t, sample);
@@ -249,7 +301,7 @@ static bool fine_near(const qm_terrain& T, const double a[2], const double b[2])
     const double ys[2] = {std::min(a[1], b[1]), std::max(a[1], b[1])};
     for (double x : xs)
         for (double y : ys)
-            if (fine_block(T, (x - T.origin_x_m) / T.spacing_m, (y - T.origin_y_m) / T.spacing_m)) return true;
+            if (fine_block(T, column_position(T, x), row_position(T, y))) return true;
     return false;
 }
 
@@ -317,7 +369,7 @@ static double snap(double steps) {
 }
 void steps_of(const qm_obstacles& O, double x, double y, double out[2]) {
     out[0] = snap(O.origin_steps_x + x / O.metres_per_step);
-    out[1] = snap(O.origin_steps_y - y / O.metres_per_step);
+    out[1] = snap(O.origin_steps_y - y / (O.metres_per_step_y > 0.0 ? O.metres_per_step_y : O.metres_per_step));
 }
 
 // crossings.rs:13-28
@@ -500,15 +552,40 @@ void crossings_brute_force(const qm_obstacles& O, const double from[2], const do
 }
 
 // ---------------------------------------------------------------------------------------------
-// physics/src/weather.rs:175-214 corners + place_of on the planar weather lattice (clamped at its
-// edges where the globe's table wraps in longitude).
+// physics/src/weather.rs:175-214 corners + place_of: on the world table by the place's latitude
+// and longitude (the geographic frame), or on a planar weather lattice (clamped at its edges).
 void place_weather(const qm_weather& W, double x, double y, double favourable[PERIODS][SECTORS], double alpha[BANDS]) {
-    const double yy = std::min(std::max((y - W.origin_y_m) / W.spacing_m, 0.0), (double)(W.rows - 1));
-    const double xx = std::min(std::max((x - W.origin_x_m) / W.spacing_m, 0.0), (double)(W.columns - 1));
-    const uint32_t row = std::min((uint32_t)std::floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
-    const uint32_t column = std::min((uint32_t)std::floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
-    const double fy = yy - row, fx = xx - column;
-    const uint32_t r1 = W.rows >= 2 ? row + 1 : row, c1 = W.columns >= 2 ? column + 1 : column;
+    uint32_t row, column, r1, c1;
+    double fy, fx;
+    if (W.nodes_per_degree) {
+        // geo::Mercator::to_degrees of the frame point (z12 units, 4096 a side), then the world
+        // table's corners (weather.rs:186-200), longitude wrapping.
+        const double n = 4096.0;
+        const double mx = W.frame_x + x / W.east_m_per_unit, my = W.frame_y - y / W.north_m_per_unit;
+        const double lat = std::atan(std::sinh(M_PI * (1.0 - 2.0 * my / n))) * (180.0 / M_PI);
+        const double lon = mx / n * 360.0 - 180.0;
+        const double nodes = (double)W.nodes_per_degree;
+        const double yy = std::min(std::max((90.0 - lat) * nodes, 0.0), (double)(W.rows - 1));
+        double wrapped = std::fmod(lon, 360.0);
+        if (wrapped < 0.0) wrapped += 360.0;
+        const double xx = wrapped * nodes;
+        row = std::min((uint32_t)std::floor(yy), W.rows - 2);
+        const uint32_t at = (uint32_t)std::floor(xx);
+        fy = yy - row;
+        fx = xx - at;
+        column = at % W.columns;
+        c1 = (at + 1) % W.columns;
+        r1 = row + 1;
+    } else {
+        const double yy = std::min(std::max((y - W.origin_y_m) / W.spacing_m, 0.0), (double)(W.rows - 1));
+        const double xx = std::min(std::max((x - W.origin_x_m) / W.spacing_m, 0.0), (double)(W.columns - 1));
+        row = std::min((uint32_t)std::floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
+        column = std::min((uint32_t)std::floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
+        fy = yy - row;
+        fx = xx - column;
+        r1 = W.rows >= 2 ? row + 1 : row;
+        c1 = W.columns >= 2 ? column + 1 : column;
+    }
     const uint8_t* n[4] = {W.nodes + ((size_t)row * W.columns + column) * QM_WEATHER_NODE_BYTES,
                            W.nodes + ((size_t)row * W.columns + c1) * QM_WEATHER_NODE_BYTES,
                            W.nodes + ((size_t)r1 * W.columns + column) * QM_WEATHER_NODE_BYTES,

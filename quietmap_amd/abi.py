@@ -84,6 +84,8 @@ class CTerrain(C.Structure):
         ("canopy_m", C.c_void_p),
         ("fine_detail", C.c_void_p),
         ("fine_blocks", C.c_void_p),
+        ("column_m", C.c_double),
+        ("row_y_m", C.c_void_p),
     ]
 
 
@@ -110,6 +112,7 @@ class CObstacles(C.Structure):
         ("runs", C.c_void_p),
         ("outlines", C.c_void_p),
         ("vertices", C.c_void_p),
+        ("metres_per_step_y", C.c_double),
     ]
 
 
@@ -121,6 +124,12 @@ class CWeather(C.Structure):
         ("columns", C.c_uint32),
         ("rows", C.c_uint32),
         ("nodes", C.c_void_p),
+        ("nodes_per_degree", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("frame_x", C.c_double),
+        ("frame_y", C.c_double),
+        ("east_m_per_unit", C.c_double),
+        ("north_m_per_unit", C.c_double),
     ]
 
 
@@ -129,9 +138,11 @@ class Terrain:
     forest << 24, ``canopy_m`` (rows, columns) u8; row r at y = origin_y + r*spacing. Optional
     fine terrain: ``fine_detail`` (FINE_PER_NODE*(rows-1)+1, FINE_PER_NODE*(columns-1)+1) i16 steps
     of DETAIL_STEP_M and ``fine_blocks`` (block rows, block columns) u8 flags of FINE_BLOCK_CELLS
-    square blocks."""
+    square blocks. The geographic frame: ``row_y_m`` (rows ascending north metres of each row) and
+    ``column_m`` (east metres between columns), both from the square's Mercator frame."""
 
-    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, canopy_m=None, fine_detail=None, fine_blocks=None):
+    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, canopy_m=None, fine_detail=None, fine_blocks=None,
+                 column_m=0.0, row_y_m=None):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         rows, columns = self.nodes.shape
         self.canopy_m = (
@@ -144,6 +155,11 @@ class Terrain:
             assert self.fine_detail.shape == (FINE_PER_NODE * (rows - 1) + 1, FINE_PER_NODE * (columns - 1) + 1)
             self.fine_blocks = np.ascontiguousarray(fine_blocks, np.uint8)
             assert self.fine_blocks.shape == ((rows - 2) // FINE_BLOCK_CELLS + 1, (columns - 2) // FINE_BLOCK_CELLS + 1)
+        self.column_m = float(column_m)
+        self.row_y_m = None
+        if row_y_m is not None:
+            self.row_y_m = np.ascontiguousarray(row_y_m, np.float64)
+            assert self.row_y_m.shape == (rows,) and np.all(np.diff(self.row_y_m) > 0)
         self.c = CTerrain(
             self.origin_x_m,
             self.origin_y_m,
@@ -154,6 +170,8 @@ class Terrain:
             self.canopy_m.ctypes.data,
             None if self.fine_detail is None else self.fine_detail.ctypes.data,
             None if self.fine_blocks is None else self.fine_blocks.ctypes.data,
+            self.column_m,
+            None if self.row_y_m is None else self.row_y_m.ctypes.data,
         )
 
     @staticmethod
@@ -168,7 +186,8 @@ class Terrain:
 class Obstacles:
     """The obstacle cell index (see qm_scene.h)."""
 
-    def __init__(self, origin_steps, metres_per_step, cell_origin, cells, cell_first_run, runs, outlines, vertices):
+    def __init__(self, origin_steps, metres_per_step, cell_origin, cells, cell_first_run, runs, outlines, vertices,
+                 metres_per_step_y=0.0):
         self.cell_first_run = np.ascontiguousarray(cell_first_run, np.uint32)
         self.runs = np.ascontiguousarray(runs, RUN_DTYPE)
         self.outlines = np.ascontiguousarray(outlines, OUTLINE_DTYPE)
@@ -194,17 +213,27 @@ class Obstacles:
             self.runs.ctypes.data if self.runs.size else None,
             self.outlines.ctypes.data if self.outlines.size else None,
             self.vertices.ctypes.data if self.vertices.size else None,
+            float(metres_per_step_y),
         )
+        self.metres_per_step_y = float(metres_per_step_y)
 
 
 class Weather:
     """A weather lattice of 64-byte nodes (physics/src/weather.rs:19-27)."""
 
-    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes):
+    def __init__(self, origin_x_m, origin_y_m, spacing_m, nodes, frame=None, nodes_per_degree=0):
+        """``frame`` (Mercator x, y of the frame's origin in z12 units, east and north metres per
+        unit) with ``nodes_per_degree``: the world's table read by latitude and longitude."""
         self.nodes = np.ascontiguousarray(nodes, np.uint8)
         rows, columns, width = self.nodes.shape
         assert width == WEATHER_NODE_BYTES
-        self.c = CWeather(float(origin_x_m), float(origin_y_m), float(spacing_m), columns, rows, self.nodes.ctypes.data)
+        self.frame = frame
+        self.nodes_per_degree = int(nodes_per_degree)
+        fx, fy, em, nm = frame if frame is not None else (0.0, 0.0, 0.0, 0.0)
+        if self.nodes_per_degree:
+            assert rows == 180 * self.nodes_per_degree + 1 and columns == 360 * self.nodes_per_degree
+        self.c = CWeather(float(origin_x_m), float(origin_y_m), float(spacing_m), columns, rows, self.nodes.ctypes.data,
+                          self.nodes_per_degree, 0, float(fx), float(fy), float(em), float(nm))
 
     @staticmethod
     def node(percent, alpha_db_per_km):

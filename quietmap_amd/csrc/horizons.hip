@@ -133,7 +133,8 @@ __global__ __launch_bounds__(64) void horizons_kernel(Scene sc, const qm_receive
         for (int k = 0; k < TERRAIN_SAMPLES; ++k) {
             double h;
             const double px = x + east * range, py = y + north * range;
-            if (height_at(T, (px - T.origin_x_m) / T.spacing_m, (py - T.origin_y_m) / T.spacing_m, h) == 0)
+            if (height_at(T, T.column_m > 0.0 ? column_position(T, px) : (px - T.origin_x_m) / T.spacing_m,
+                          T.row_y_m ? row_position<true>(T, py) : (py - T.origin_y_m) / T.spacing_m, h) == 0)
                 s.offer(band_of(range, TERRAIN_LIMITS), range, (h - altitude) / range);
             range *= growth;
         }
@@ -164,7 +165,8 @@ __global__ __launch_bounds__(64) void horizons_kernel(Scene sc, const qm_receive
                 if (range <= NEAREST_EDGE_M) continue;
                 double h;
                 const double px = x + east * range, py = y + north * range;
-                if (height_at(T, (px - T.origin_x_m) / T.spacing_m, (py - T.origin_y_m) / T.spacing_m, h)) {
+                if (height_at(T, T.column_m > 0.0 ? column_position(T, px) : (px - T.origin_x_m) / T.spacing_m,
+                          T.row_y_m ? row_position<true>(T, py) : (py - T.origin_y_m) / T.spacing_m, h)) {
                     failed |= QM_FAILED_NO_TERRAIN;
                     continue;
                 }

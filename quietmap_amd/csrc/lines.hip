@@ -41,10 +41,9 @@ __global__ __launch_bounds__(256) void line_nodes_kernel(Scene sc, const qm_rece
     if (rcv.failed) return;
     // candidate.traced_ground_m: the ground at the piece's ends.
     const qm_terrain& T = sc.terrain;
-    const double inv = 1.0 / T.spacing_m;
     GroundSample ga, gb;
-    uint32_t f = ground_at(T, (piece.ax_m - T.origin_x_m) * inv, (piece.ay_m - T.origin_y_m) * inv, ga);
-    f |= ground_at(T, (piece.bx_m - T.origin_x_m) * inv, (piece.by_m - T.origin_y_m) * inv, gb);
+    uint32_t f = ground_at(T, column_position(T, piece.ax_m), row_position<true>(T, piece.ay_m), ga);
+    f |= ground_at(T, column_position(T, piece.bx_m), row_position<true>(T, piece.by_m), gb);
     if (f) {
         failed[i] = f;
         return;
@@ -327,7 +326,7 @@ extern "C" int qm_evaluate_line_pairs(qm_gpu* gpu, const qm_square* square, cons
                        d_ray_pair);
     L_TRY(hipGetLastError());
     if (total) {
-        if (square->scene.terrain.fine_detail)
+        if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
             hipLaunchKernelGGL(line_rays_kernel<true>, dim3((unsigned)((total + QM_PATH_GROUP - 1) / QM_PATH_GROUP)), dim3(QM_PATH_GROUP), 0, 0,
                                square->d_scene, square->d_bands, d_receivers, d_pieces, d_pp, d_pl, d_ray_pair, d_ray_first,
                                (uint32_t)total, d_nodes, d_div, d_sums, d_failed, d_pair_bands);

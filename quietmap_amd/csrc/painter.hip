@@ -69,8 +69,8 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 // faster than at 2 (256 VGPRs, 7.6 KB) on the 1024 x 1024 receiver step.
 
 // Gpu::evaluate_pairs' ray transfer (one ray per point pair): sums and point bands.
-// FINE: the square holds fine terrain (its own instantiation, so the coarse kernel keeps its
-// registers).
+// FINE: the square holds fine terrain or the geographic frame's row table (the general
+// instantiation, so the coarse kernel keeps its registers).
 template <bool FINE>
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
@@ -320,7 +320,8 @@ int launch_paint(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_rece
     int cus = 0;
     QM_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, gpu->ordinal));
     int per_cu = 0;
-    const bool fine = square->scene.terrain.fine_detail != nullptr;
+    // The general instantiation: fine terrain or the geographic frame's row table.
+    const bool fine = square->scene.terrain.fine_detail != nullptr || square->scene.terrain.row_y_m != nullptr;
     QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, fine ? reinterpret_cast<const void*>(paint_kernel<true>) : reinterpret_cast<const void*>(paint_kernel<false>),
         GROUP, 0));
@@ -378,8 +379,15 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     if (!gpu || !terrain || !obstacles || !weather) return fail(err, errlen, "null argument");
     if (terrain->rows < 2 || terrain->columns < 2 || !(terrain->spacing_m > 0.0))
         return fail(err, errlen, "terrain lattice must have 2x2 nodes and a positive spacing");
-    if (weather->rows < 1 || weather->columns < 1 || !(weather->spacing_m > 0.0))
+    if (weather->nodes_per_degree) {
+        if (weather->rows != 180u * weather->nodes_per_degree + 1 || weather->columns != 360u * weather->nodes_per_degree)
+            return fail(err, errlen, "world weather table must be (180 npd + 1) x 360 npd nodes");
+        if (!(weather->east_m_per_unit > 0.0) || !(weather->north_m_per_unit > 0.0) ||
+            !std::isfinite(weather->frame_x) || !std::isfinite(weather->frame_y))
+            return fail(err, errlen, "weather frame must have positive metres per unit and a finite origin");
+    } else if (weather->rows < 1 || weather->columns < 1 || !(weather->spacing_m > 0.0)) {
         return fail(err, errlen, "weather lattice must have a node and a positive spacing");
+    }
     if (!(obstacles->metres_per_step > 0.0) || obstacles->cells_x == 0 || obstacles->cells_y == 0)
         return fail(err, errlen, "obstacle grid must have cells and a positive step");
     if (!(bucket_m > 0.0)) return fail(err, errlen, "bucket size must be positive");
@@ -408,6 +416,17 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     sq->scene.terrain.canopy_m = d_canopy;
     sq->scene.terrain.fine_detail = nullptr;
     sq->scene.terrain.fine_blocks = nullptr;
+    sq->scene.terrain.row_y_m = nullptr;
+    if (!(terrain->column_m >= 0.0) || !std::isfinite(terrain->column_m))
+        return cleanup(fail(err, errlen, "terrain column spacing must be finite and non-negative"));
+    if (terrain->row_y_m) {
+        for (uint32_t r = 0; r < terrain->rows; ++r)
+            if (!std::isfinite(terrain->row_y_m[r]) || (r && !(terrain->row_y_m[r] > terrain->row_y_m[r - 1])))
+                return cleanup(fail(err, errlen, "terrain row table not finite and ascending at row %u", r));
+        double* d_rows;
+        if (upload(sq, terrain->row_y_m, terrain->rows, &d_rows, err, errlen)) return cleanup(1);
+        sq->scene.terrain.row_y_m = d_rows;
+    }
     if (terrain->fine_detail) {
         if (!terrain->fine_blocks) return cleanup(fail(err, errlen, "fine terrain without its blocks"));
         const size_t fine = (size_t)(QM_FINE_PER_NODE * (terrain->rows - 1) + 1) *
@@ -578,7 +597,7 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                           errlen))
         return 1;
     const unsigned blocks = (unsigned)((n_pairs + 255) / 256);
-    if (square->scene.terrain.fine_detail)
+    if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
         hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
                            (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
                            (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,

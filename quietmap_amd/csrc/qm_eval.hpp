@@ -198,11 +198,10 @@ struct StretchPlan {
 
 // popup/src/scene.rs Ground::fine_near: a block at a corner of the stretch's box has fine terrain.
 QM_DEV bool fine_near(const qm_terrain& T, double ax, double ay, double bx, double by) {
-    const double inv = 1.0 / T.spacing_m;
     const double xs[2] = {fmin(ax, bx), fmax(ax, bx)}, ys[2] = {fmin(ay, by), fmax(ay, by)};
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-            if (fine_block(T, (xs[i] - T.origin_x_m) * inv, (ys[j] - T.origin_y_m) * inv)) return true;
+            if (fine_block(T, column_position(T, xs[i]), row_position<true>(T, ys[j]))) return true;
     return false;
 }
 
@@ -240,9 +239,8 @@ QM_DEV StretchPlan stretch_plan(const qm_terrain& T, double sx, double sy, doubl
 // The switch vertex at fraction `to` of the ray (Ground::at where the lattice changes).
 QM_DEV uint32_t switch_vertex(const qm_terrain& T, double sx, double sy, double rx, double ry, double to,
                               GroundSample& out) {
-    const double inv = 1.0 / T.spacing_m;
     const double x = sx - to * (sx - rx), y = sy - to * (sy - ry);
-    return ground_at_fine(T, (x - T.origin_x_m) * inv, (y - T.origin_y_m) * inv, out);
+    return ground_at_fine(T, column_position(T, x), row_position<true>(T, y), out);
 }
 
 // The ray's ends and stretches, for terrain_bracket on a fine scene.
@@ -267,7 +265,7 @@ QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, 
         int k = 0;
         while (k + 1 < P.count && !(t < P.to[k])) ++k;
         const float from = k ? P.to[k - 1] : 0.0f, to = P.to[k];
-        const Lattice Lk = lattice_of(T, ray->sx, ray->sy, ray->rx, ray->ry, P.scale[k], from);
+        const Lattice Lk = lattice_of(T, ray->sx, ray->sy, ray->rx, ray->ry, P.scale[k], from, true);
         failed = bracket_vertex<true>(T, Lk, t, length, false, tp, a, ea, from, to);
         failed |= bracket_vertex<true>(T, Lk, t, length, true, tn, b, eb, from, to);
         if (ea) {
@@ -279,8 +277,8 @@ QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, 
             else failed |= switch_vertex(T, ray->sx, ray->sy, ray->rx, ray->ry, P.to_d[k], b);
         }
     } else {
-        failed = bracket_vertex(T, L, t, length, false, tp, a, ea);
-        failed |= bracket_vertex(T, L, t, length, true, tn, b, eb);
+        failed = bracket_vertex<FINE>(T, L, t, length, false, tp, a, ea);
+        failed |= bracket_vertex<FINE>(T, L, t, length, true, tn, b, eb);
         if (ea) a = start;
         if (eb) b = end;
     }
@@ -303,16 +301,40 @@ QM_DEV qm_receiver receiver_of(const Scene& sc, const qm_point& p) {
     r.own_footprint = p.own_footprint;
     const qm_terrain& T = sc.terrain;
     GroundSample g;
-    const double inv = 1.0 / T.spacing_m;
-    r.failed = ground_at_fine(T, (p.x_m - T.origin_x_m) * inv, (p.y_m - T.origin_y_m) * inv, g);
+    r.failed = ground_at_fine(T, column_position(T, p.x_m), row_position<true>(T, p.y_m), g);
     r.altitude_m = r.failed ? 0.0f : g.z + (float)QM_RECEIVER_HEIGHT_M;
     const qm_weather& W = sc.weather;
-    const double yy = clampd((p.y_m - W.origin_y_m) / W.spacing_m, 0.0, (double)(W.rows - 1));
-    const double xx = clampd((p.x_m - W.origin_x_m) / W.spacing_m, 0.0, (double)(W.columns - 1));
-    const uint32_t row = min((uint32_t)floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
-    const uint32_t col = min((uint32_t)floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
-    const float fy = (float)(yy - row), fx = (float)(xx - col);
-    const uint32_t r1 = W.rows >= 2 ? row + 1 : row, c1 = W.columns >= 2 ? col + 1 : col;
+    uint32_t row, col, r1, c1;
+    float fy, fx;
+    if (W.nodes_per_degree) {
+        // The place's latitude and longitude (geo::Mercator::to_degrees), then the world table's
+        // four nodes about it, longitude wrapping (weather.rs corners).
+        const double n = 4096.0;
+        const double mx = W.frame_x + p.x_m / W.east_m_per_unit, my = W.frame_y - p.y_m / W.north_m_per_unit;
+        const double lat = atan(sinh(3.14159265358979323846 * (1.0 - 2.0 * my / n))) * (180.0 / 3.14159265358979323846);
+        const double lon = mx / n * 360.0 - 180.0;
+        const double npd = (double)W.nodes_per_degree;
+        const double yy = clampd((90.0 - lat) * npd, 0.0, (double)(W.rows - 1));
+        double lw = fmod(lon, 360.0);
+        if (lw < 0.0) lw += 360.0;
+        const double xx = lw * npd;
+        row = min((uint32_t)floor(yy), W.rows - 2);
+        const uint32_t column = (uint32_t)floor(xx);
+        fy = (float)(yy - row);
+        fx = (float)(xx - column);
+        col = column % W.columns;
+        c1 = (column + 1) % W.columns;
+        r1 = row + 1;
+    } else {
+        const double yy = clampd((p.y_m - W.origin_y_m) / W.spacing_m, 0.0, (double)(W.rows - 1));
+        const double xx = clampd((p.x_m - W.origin_x_m) / W.spacing_m, 0.0, (double)(W.columns - 1));
+        row = min((uint32_t)floor(yy), W.rows >= 2 ? W.rows - 2 : 0u);
+        col = min((uint32_t)floor(xx), W.columns >= 2 ? W.columns - 2 : 0u);
+        fy = (float)(yy - row);
+        fx = (float)(xx - col);
+        r1 = W.rows >= 2 ? row + 1 : row;
+        c1 = W.columns >= 2 ? col + 1 : col;
+    }
     const uint8_t* nd[4] = {W.nodes + ((size_t)row * W.columns + col) * QM_WEATHER_NODE_BYTES,
                             W.nodes + ((size_t)row * W.columns + c1) * QM_WEATHER_NODE_BYTES,
                             W.nodes + ((size_t)r1 * W.columns + col) * QM_WEATHER_NODE_BYTES,
@@ -372,17 +394,16 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     }
 
     // The ends (tiles/src/terrain.rs:342-385 at both ends of popup/src/scene.rs:242-309).
-    const Lattice L = lattice_of(T, sx, sy, rx, ry);
-    const double inv = 1.0 / T.spacing_m;
+    const Lattice L = lattice_of(T, sx, sy, rx, ry, 1, 0.0f, FINE);
     GroundSample s0, r1;
     RayEnds ray;
     if (FINE) {
         ray = RayEnds{sx, sy, rx, ry, stretch_plan(T, sx, sy, rx, ry, length_d)};
-        failed |= ground_at_fine(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
-        failed |= ground_at_fine(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+        failed |= ground_at_fine(T, column_position(T, sx), row_position<true>(T, sy), s0);
+        failed |= ground_at_fine(T, column_position(T, rx), row_position<true>(T, ry), r1);
     } else {
-        failed |= ground_at(T, (sx - T.origin_x_m) * inv, (sy - T.origin_y_m) * inv, s0);
-        failed |= ground_at(T, (rx - T.origin_x_m) * inv, (ry - T.origin_y_m) * inv, r1);
+        failed |= ground_at(T, column_position(T, sx), row_position(T, sy), s0);
+        failed |= ground_at(T, column_position(T, rx), row_position(T, ry), r1);
     }
     if (failed & QM_FAILED_NO_TERRAIN) {
         out.failed = failed;
@@ -521,14 +542,14 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     int stretch = 0;
     float stretch_to = 1.0f;
     if (FINE && !(ray.plan.count == 1 && ray.plan.scale[0] == 1)) {
-        WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[0], 0.0f);
+        WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[0], 0.0f, true);
         stretch_to = ray.plan.to[0];
     }
     int next_column = WL.first_column, next_row = WL.first_row, south = WL.start_south;
     const float inf = __builtin_inff();
     // Every line crossed is one vertex: a stretch ends within |columns| + |rise| + 2 steps of its
     // lattice over its share of the ray.
-    int budget = (int)((fabsf(WL.columns) + fabsf(WL.rise)) * stretch_to) + 4;
+    int budget = (int)(WL.span * stretch_to) + 4;
     bool last = false;
     while (!last) {
         if (--budget < 0) {
@@ -552,11 +573,11 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
                 const float from = stretch_to;
                 ++stretch;
                 stretch_to = ray.plan.to[stretch];
-                WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[stretch], from);
+                WL = lattice_of(T, sx, sy, rx, ry, ray.plan.scale[stretch], from, true);
                 next_column = WL.first_column;
                 next_row = WL.first_row;
                 south = WL.start_south;
-                budget += (int)((fabsf(WL.columns) + fabsf(WL.rise)) * (stretch_to - from)) + 4;
+                budget += (int)(WL.span * (stretch_to - from)) + 4;
                 if (failed & QM_FAILED_NO_TERRAIN) break;
             }
         } else {

@@ -109,13 +109,14 @@ QM_DEV void skyline_mark(const qm_obstacles& O, double x, double y, double lo, d
                          double los_floor, const LineGeometry& g, double need_radius, double span_lo, double span_hi,
                          double bin_width, Mask& m) {
     const double mps = O.metres_per_step;
-    const double ox = O.origin_steps_x + x / mps, oy = O.origin_steps_y - y / mps;
+    const double mpy = O.metres_per_step_y > 0.0 ? O.metres_per_step_y : mps;
+    const double ox = O.origin_steps_x + x / mps, oy = O.origin_steps_y - y / mpy;
     const double cell = (double)QM_CELL_STEPS;
     const double lox = cos(lo), loy = sin(lo), hix = cos(hi), hiy = sin(hi);
-    const long long row0 = (long long)floor((oy - radius / mps) / cell), row1 = (long long)floor((oy + radius / mps) / cell);
+    const long long row0 = (long long)floor((oy - radius / mpy) / cell), row1 = (long long)floor((oy + radius / mpy) / cell);
     const long long col0 = (long long)floor((ox - radius / mps) / cell), col1 = (long long)floor((ox + radius / mps) / cell);
     for (long long row = row0; row <= row1; ++row) {
-        const double south = (oy - (row + 1) * cell) * mps, north = (oy - row * cell) * mps;
+        const double south = (oy - (row + 1) * cell) * mpy, north = (oy - row * cell) * mpy;
         const double dy = fmax(fmax(south, -north), 0.0);
         for (long long column = col0; column <= col1; ++column) {
             const double west = (column * cell - ox) * mps, east = ((column + 1) * cell - ox) * mps;
@@ -140,11 +141,11 @@ QM_DEV void skyline_mark(const qm_obstacles& O, double x, double y, double lo, d
                 const qm_outline& record = O.outlines[run.outline];
                 if (record.kind != QM_KIND_BUILDING && record.height_dm * 0.1 <= los_floor) continue;
                 const uint32_t first = record.first_vertex + run.first_edge;
-                double ax = ((double)O.vertices[2 * first] - ox) * mps, ay = (oy - (double)O.vertices[2 * first + 1]) * mps;
+                double ax = ((double)O.vertices[2 * first] - ox) * mps, ay = (oy - (double)O.vertices[2 * first + 1]) * mpy;
                 double azimuth_a = atan2(ay, ax);
                 for (uint32_t next = first + 1; next <= first + run.edge_count; ++next) {
                     const double bx = ((double)O.vertices[2 * next] - ox) * mps;
-                    const double by = (oy - (double)O.vertices[2 * next + 1]) * mps;
+                    const double by = (oy - (double)O.vertices[2 * next + 1]) * mpy;
                     const double azimuth_b = atan2(by, bx);
                     // origin_to_segment_distance (skyline.rs:8-20)
                     const double ex = bx - ax, ey = by - ay, length2 = ex * ex + ey * ey;

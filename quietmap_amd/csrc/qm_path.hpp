@@ -113,36 +113,107 @@ QM_DEV uint32_t ground_at(const qm_terrain& T, double cp, double rp, GroundSampl
     return 0;
 }
 
+// The geographic frame (qm_terrain.row_y_m, column_m): the one arc-second rows lie at the table's
+// north metres (uneven: latitude through Mercator, popup/src/scene.rs:303-330 row_line_y and
+// rows_north_of), the columns evenly at column_m. GEO: the caller may meet such a table (the
+// general instantiations); without one, or with GEO false, row r lies at origin_y + r * spacing.
+QM_DEV double column_metres(const qm_terrain& T) { return T.column_m > 0.0 ? T.column_m : T.spacing_m; }
+
+// North metres of row k of the table, the end spacings extended beyond it.
+QM_DEV double table_row_y(const qm_terrain& T, int k) {
+    const int last = (int)T.rows - 1;
+    if (k < 0) return T.row_y_m[0] + (double)k * (T.row_y_m[1] - T.row_y_m[0]);
+    if (k > last) return T.row_y_m[last] + (double)(k - last) * (T.row_y_m[last] - T.row_y_m[last - 1]);
+    return T.row_y_m[k];
+}
+
+// The fractional column and row (node indices) of frame metres x and y.
+QM_DEV double column_position(const qm_terrain& T, double x) { return (x - T.origin_x_m) * (1.0 / column_metres(T)); }
+template <bool GEO = false>
+QM_DEV double row_position(const qm_terrain& T, double y) {
+    if (!GEO || !T.row_y_m) return (y - T.origin_y_m) * (1.0 / T.spacing_m);
+    const int last = (int)T.rows - 1;
+    int lo = 0;
+    if (y < T.row_y_m[0]) {
+        lo = 0;
+    } else if (y >= T.row_y_m[last]) {
+        lo = last - 1;
+    } else {
+        int hi = last;  // row_y_m[lo] <= y < row_y_m[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (T.row_y_m[mid] <= y) lo = mid;
+            else hi = mid;
+        }
+    }
+    return (double)lo + (y - T.row_y_m[lo]) / (T.row_y_m[lo + 1] - T.row_y_m[lo]);
+}
+
 // The ray's lattice geometry: position of the source (ca, ra) in lattice units, the offsets to
 // the receiver, and the first lines strictly ahead of the source.
 // `scale`: the lattice scale times as fine as the coarse one (1, or QM_FINE_PER_NODE on a fine
 // stretch); the first lines are those strictly after fraction `from` of the ray.
+// With a row table (`geo`), the row units are north metres from the source (ra 0, rise the ray's
+// north extent) and row line k of the lattice lies on the table, a fine line evenly between the
+// table's rows.
 struct Lattice {
     float ca, ra, columns, rise, inv_columns, inv_rise;
     int first_column, column_step, first_row, row_step, start_south;
     int scale;
+    float span;  // lines the ray crosses, both families (the walk's step budget)
+    bool geo;
+    const qm_terrain* terrain;
+    double y0, dy;
 };
 
+// North metres of row line k of a geo lattice.
+QM_DEV double row_line_y(const Lattice& L, int k) {
+    const int s = L.scale;
+    const int coarse = k >= 0 ? k / s : -((-k + s - 1) / s);
+    const int sub = k - coarse * s;
+    const double y0 = table_row_y(*L.terrain, coarse);
+    if (sub == 0) return y0;
+    return y0 + (table_row_y(*L.terrain, coarse + 1) - y0) * (double)sub / (double)s;
+}
+
 QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, double ry, int scale = 1,
-                          float from = 0.0f) {
+                          float from = 0.0f, bool geo_ok = false) {
     Lattice L;
     L.scale = scale;
-    const double inv = (double)scale / T.spacing_m;
+    L.geo = geo_ok && T.row_y_m != nullptr;
+    L.terrain = &T;
+    const double inv = (double)scale / column_metres(T);
     const double ca = (sx - T.origin_x_m) * inv, cb = (rx - T.origin_x_m) * inv;
-    const double ra = (sy - T.origin_y_m) * inv, rb = (ry - T.origin_y_m) * inv;
     L.ca = (float)ca;
-    L.ra = (float)ra;
     L.columns = (float)(cb - ca);
-    L.rise = (float)(rb - ra);
     L.inv_columns = L.columns != 0.0f ? 1.0f / L.columns : 0.0f;
-    L.inv_rise = L.rise != 0.0f ? 1.0f / L.rise : 0.0f;
-    const float c0 = L.ca + from * L.columns, r0 = L.ra + from * L.rise;
+    const float c0 = L.ca + from * L.columns;
     if (L.columns > 0.0f) {
         L.first_column = (int)floorf(c0) + 1;
         L.column_step = 1;
     } else {
         L.first_column = (int)ceilf(c0) - 1;
         L.column_step = -1;
+    }
+    float r0;
+    if (L.geo) {
+        L.y0 = sy;
+        L.dy = ry - sy;
+        L.ra = 0.0f;
+        L.rise = (float)L.dy;
+        L.inv_rise = L.rise != 0.0f ? 1.0f / L.rise : 0.0f;
+        const double pa = (double)scale * row_position<true>(T, sy), pb = (double)scale * row_position<true>(T, ry);
+        r0 = (float)((double)scale * row_position<true>(T, sy + (double)from * L.dy));
+        L.span = fabsf(L.columns) + (float)fabs(pb - pa);
+    } else {
+        const double rinv = (double)scale / T.spacing_m;
+        const double ra = (sy - T.origin_y_m) * rinv, rb = (ry - T.origin_y_m) * rinv;
+        L.ra = (float)ra;
+        L.rise = (float)(rb - ra);
+        L.inv_rise = L.rise != 0.0f ? 1.0f / L.rise : 0.0f;
+        r0 = L.ra + from * L.rise;
+        L.y0 = L.dy = 0.0;
+        L.span = fabsf(L.columns) + fabsf(L.rise);
     }
     if (L.rise > 0.0f) {
         L.first_row = (int)floorf(r0) + 1;
@@ -158,6 +229,12 @@ QM_DEV Lattice lattice_of(const qm_terrain& T, double sx, double sy, double rx, 
         L.start_south = (int)floorf(r0);
     }
     return L;
+}
+
+// The fractional row line (this lattice's units) of the ray at fraction t.
+QM_DEV float row_units_at(const Lattice& L, float t) {
+    if (L.geo) return (float)((double)L.scale * row_position<true>(*L.terrain, L.y0 + (double)t * L.dy));
+    return L.ra + t * L.rise;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -226,7 +303,10 @@ QM_DEV uint32_t fine_vertex(const qm_terrain& T, bool on_row, int line, int sout
 }
 
 QM_DEV float t_of_column(const Lattice& L, int c) { return ((float)c - L.ca) * L.inv_columns; }
-QM_DEV float t_of_row(const Lattice& L, int k) { return ((float)k - L.ra) * L.inv_rise; }
+QM_DEV float t_of_row(const Lattice& L, int k) {
+    if (L.geo) return (float)((row_line_y(L, k) - L.y0) / L.dy);
+    return ((float)k - L.ra) * L.inv_rise;
+}
 
 // Sample on row line k at fraction t (the fine sample on a fine lattice when FINE).
 template <bool FINE = false>
@@ -240,8 +320,14 @@ QM_DEV uint32_t row_vertex(const qm_terrain& T, const Lattice& L, int k, float t
 template <bool FINE = false>
 QM_DEV uint32_t column_vertex(const qm_terrain& T, const Lattice& L, int c, int south, float t, GroundSample& out) {
     const float r = L.ra + t * L.rise;
-    if (FINE && L.scale != 1) return fine_vertex(T, false, c, south, r, clampf(r - (float)south, 0.0f, 1.0f), out);
-    return edge_sample(T, south, c, south + 1, c, clampf(r - (float)south, 0.0f, 1.0f), out);
+    float f = clampf(r - (float)south, 0.0f, 1.0f);
+    if (FINE && L.geo) {
+        // Between the row lines about the vertex, by north metres (scene.rs walk's fraction).
+        const double y = L.y0 + (double)t * L.dy, y0 = row_line_y(L, south), y1 = row_line_y(L, south + 1);
+        f = (float)clampd((y - y0) / (y1 - y0), 0.0, 1.0);
+    }
+    if (FINE && L.scale != 1) return fine_vertex(T, false, c, south, r, f, out);
+    return edge_sample(T, south, c, south + 1, c, f, out);
 }
 
 // The walk's terrain vertex nearest before (or after) fraction t, directly: popup/src/scene.rs's
@@ -280,7 +366,7 @@ QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, f
         }
     }
     if (L.rise != 0.0f) {
-        const float pos = L.ra + t * L.rise;
+        const float pos = FINE ? row_units_at(L, t) : L.ra + t * L.rise;
         int k = L.row_step > 0 ? (int)floorf(pos) : (int)ceilf(pos);
         for (int j = 0; j < 2; ++j)
             if (t_of_row(L, k) * length > x) k -= L.row_step;
@@ -304,7 +390,7 @@ QM_DEV uint32_t bracket_vertex(const qm_terrain& T, const Lattice& L, float t, f
     is_end = kind == 0;
     if (kind == 2) return row_vertex<FINE>(T, L, line, best_t, out);
     if (kind == 1) {
-        const float r = L.ra + best_t * L.rise;
+        const float r = FINE ? row_units_at(L, best_t) : L.ra + best_t * L.rise;
         const int south = L.rise >= 0.0f ? (int)floorf(r) : (int)ceilf(r) - 1;
         return column_vertex<FINE>(T, L, line, L.rise == 0.0f ? L.start_south : south, best_t, out);
     }
@@ -366,10 +452,11 @@ QM_DEV uint32_t find_crossings(const qm_obstacles& O, double fx, double fy, doub
                                int& n) {
 #pragma clang fp contract(off)
     n = 0;
+    const double mpy = O.metres_per_step_y > 0.0 ? O.metres_per_step_y : O.metres_per_step;
     const double start[2] = {snap_steps(O.origin_steps_x + fx / O.metres_per_step),
-                             snap_steps(O.origin_steps_y - fy / O.metres_per_step)};
+                             snap_steps(O.origin_steps_y - fy / mpy)};
     const double end[2] = {snap_steps(O.origin_steps_x + tx / O.metres_per_step),
-                           snap_steps(O.origin_steps_y - ty / O.metres_per_step)};
+                           snap_steps(O.origin_steps_y - ty / mpy)};
     const double delta[2] = {end[0] - start[0], end[1] - start[1]};
     const double cs = (double)QM_CELL_STEPS;
     int64_t cell[2] = {(int64_t)floor(start[0] / cs), (int64_t)floor(start[1] / cs)};

@@ -7,7 +7,9 @@
  *  - terrain: one regular lattice of decoded nodes, 4 bytes each as the reference's decoded
  *    qmterr2 node (tiles/src/terrain.rs:21-27, 149-159): height code u16 (-500 m + code/5),
  *    imperviousness u8 (G = 1 - imperviousness/100), forest cover u8, plus a canopy-height byte
- *    plane. Row r lies at y = origin_y + r*spacing, column c at x = origin_x + c*spacing.
+ *    plane. Row r lies at y = origin_y + r*spacing, column c at x = origin_x + c*spacing; in the
+ *    geographic frame (a square's one arc-second lattice, popup/src/scene.rs) row r lies at
+ *    y = row_y_m[r] (uneven: latitude through Mercator) and column c at origin_x + c*column_m.
  *  - obstacles: the cell index of buildings and walls in integer "steps" (y south), cells of 256
  *    steps, per cell the runs of consecutive edges of one outline through it, outlines with their
  *    footprint id, height (dm) and kind, vertices as exact integers (tiles/src/obstacles.rs:1-23,
@@ -70,6 +72,12 @@ typedef struct qm_terrain {
      * flag per block ((columns-2)/QM_FINE_BLOCK_CELLS + 1 blocks a row, likewise for rows). */
     const int16_t* fine_detail;
     const uint8_t* fine_blocks;
+    /* The geographic frame (0 / null: the square lattice above). `column_m`: east metres between
+     * columns; `row_y_m`: rows ascending values, the frame north metres of each row (the rows of
+     * the one arc-second lattice; DeviceSquare::upload's scene_rows, paint-gpu/src/
+     * square_upload.rs:40-56). Fine rows lie evenly between the table's. */
+    double column_m;
+    const double* row_y_m;
 } qm_terrain;
 
 typedef struct qm_run {
@@ -97,6 +105,10 @@ typedef struct qm_obstacles {
     const qm_run* runs;
     const qm_outline* outlines;
     const int32_t* vertices; /* 2 per vertex: x, y steps */
+    /* North metres per step where they differ from the east's (the geographic frame's
+     * LocalFrame, tiles/src/geo.rs:224-243: east and north metres per Mercator unit on the
+     * WGS84 ellipsoid); 0: metres_per_step both ways. */
+    double metres_per_step_y;
 } qm_obstacles;
 
 typedef struct qm_weather {
@@ -104,6 +116,14 @@ typedef struct qm_weather {
     double spacing_m;
     uint32_t columns, rows;
     const uint8_t* nodes; /* rows*columns*64 */
+    /* The geographic frame (nodes_per_degree 0: the planar lattice above). The nodes are then the
+     * world's table (physics/src/weather.rs:15-25: rows 180 npd + 1 from 90 N, columns 360 npd
+     * from 0 E, wrapping) and a place is read by its latitude and longitude (weather.rs:127-135,
+     * 186-200), the frame mapping metres to z12 Mercator units (geo::Mercator: 4096 a side;
+     * x = frame_x + east/east_m_per_unit, y = frame_y - north/north_m_per_unit; paint-gpu/kernels/
+     * paint.cu:35-41). */
+    uint32_t nodes_per_degree, pad;
+    double frame_x, frame_y, east_m_per_unit, north_m_per_unit;
 } qm_weather;
 
 /* A point source: the candidate's place and its attribute (popup/src/candidates.rs:35-49). A

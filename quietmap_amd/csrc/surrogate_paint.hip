@@ -361,8 +361,8 @@ extern "C" int qm_surrogate_samples_device(qm_gpu* gpu, const qm_square* square,
         return fail(err, errlen, "null argument");
     if (n_pairs == 0) return 0;
     if (n_pairs > 0xffffffffu) return fail(err, errlen, "too many pairs");
-    if (square->scene.terrain.fine_detail)
-        return fail(err, errlen, "fine terrain: the network is trained on the coarse walk");
+    if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
+        return fail(err, errlen, "fine terrain or a row table: the network is trained on the coarse uniform walk");
     SP_TRY(hipSetDevice(gpu->ordinal));
     hipLaunchKernelGGL(samples_kernel, dim3(blocks_for(n_pairs)), dim3(QM_PATH_GROUP), 0, (hipStream_t)stream, square->d_scene,
                        square->d_bands, d_receivers, square->d_sources_by_input, d_pair_receiver, d_pair_source,
@@ -379,8 +379,8 @@ extern "C" int qm_surrogate_paint_receivers_device(qm_gpu* gpu, const qm_square*
     if (!gpu || !square || !mlp || !d_receivers || !d_period_energy || !d_paths || !d_failures)
         return fail(err, errlen, "null argument");
     if (!(cutoff_m >= 0.0)) return fail(err, errlen, "cutoff must be non-negative");
-    if (square->scene.terrain.fine_detail)
-        return fail(err, errlen, "fine terrain: the network is trained on the coarse walk");
+    if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
+        return fail(err, errlen, "fine terrain or a row table: the network is trained on the coarse uniform walk");
     const size_t n = (size_t)nx * ny;
     if (n == 0) return 0;
     if (n > 0xffffffffu) return fail(err, errlen, "too many receivers");

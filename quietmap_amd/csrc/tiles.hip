@@ -228,8 +228,8 @@ extern "C" int qm_tiles_plan(qm_gpu* gpu, const qm_square* square, const qm_tile
         return fail(err, errlen, "null argument");
     *out = nullptr;
     if (block_sources == 0) return fail(err, errlen, "block_sources must be at least 1");
-    if (square->scene.terrain.fine_detail)
-        return fail(err, errlen, "fine terrain: paint the square with qm_paint_grid or qm_evaluate_pairs");
+    if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
+        return fail(err, errlen, "fine terrain or a row table: paint the square with qm_paint_grid or qm_evaluate_pairs");
     if (groups_per_cu == 0 || groups_per_cu > 8) return fail(err, errlen, "groups_per_cu must be 1..8");
     for (size_t t = 0; t < n_tiles; ++t) {
         const qm_tile& T = tiles[t];

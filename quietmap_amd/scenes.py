@@ -41,17 +41,21 @@ def _weather(origin, extent, percent_of, alpha=ALPHA_15C_70, spacing=1000.0):
     return abi.Weather(origin[0] - spacing / 2, origin[1] - spacing / 2, spacing, nodes)
 
 
-def _obstacles(rings, heights, kinds, footprints, origin_m, extent_m, metres_per_step=METRES_PER_STEP):
+def _obstacles(rings, heights, kinds, footprints, origin_m, extent_m, metres_per_step=METRES_PER_STEP,
+               metres_per_step_y=None):
     """The cell index of outlines given in metres: vertices snapped to integer steps (y south), each
     ring closed by repeating its first vertex, every edge listed in every cell its step box
-    (padded by one step) touches, consecutive edges of one outline in one cell joined as a run."""
-    origin_steps = (10_000.0, 10_000.0 + extent_m[1] / metres_per_step + 5_000.0)
+    (padded by one step) touches, consecutive edges of one outline in one cell joined as a run.
+    ``metres_per_step_y``: north metres per step where they differ from the east's."""
+    mpx = metres_per_step
+    mpy = metres_per_step if metres_per_step_y is None else metres_per_step_y
+    origin_steps = (10_000.0, 10_000.0 + extent_m[1] / mpy + 5_000.0)
     cs = abi.CELL_STEPS
     # Grid covering the area plus a margin of cells.
-    low_steps = (origin_steps[0] + (origin_m[0] - 2000.0) / metres_per_step,
-                 origin_steps[1] - (origin_m[1] + extent_m[1] + 2000.0) / metres_per_step)
-    high_steps = (origin_steps[0] + (origin_m[0] + extent_m[0] + 2000.0) / metres_per_step,
-                  origin_steps[1] - (origin_m[1] - 2000.0) / metres_per_step)
+    low_steps = (origin_steps[0] + (origin_m[0] - 2000.0) / mpx,
+                 origin_steps[1] - (origin_m[1] + extent_m[1] + 2000.0) / mpy)
+    high_steps = (origin_steps[0] + (origin_m[0] + extent_m[0] + 2000.0) / mpx,
+                  origin_steps[1] - (origin_m[1] - 2000.0) / mpy)
     cell_x0, cell_y0 = int(math.floor(low_steps[0] / cs)), int(math.floor(low_steps[1] / cs))
     cells_x = int(math.floor(high_steps[0] / cs)) - cell_x0 + 1
     cells_y = int(math.floor(high_steps[1] / cs)) - cell_y0 + 1
@@ -61,8 +65,8 @@ def _obstacles(rings, heights, kinds, footprints, origin_m, extent_m, metres_per
     first = 0
     for o, ring in enumerate(rings):
         ring = np.asarray(ring, np.float64)
-        sx = np.round(origin_steps[0] + ring[:, 0] / metres_per_step).astype(np.int64)
-        sy = np.round(origin_steps[1] - ring[:, 1] / metres_per_step).astype(np.int64)
+        sx = np.round(origin_steps[0] + ring[:, 0] / mpx).astype(np.int64)
+        sy = np.round(origin_steps[1] - ring[:, 1] / mpy).astype(np.int64)
         if kinds[o] == abi.KIND_BUILDING:
             sx, sy = np.append(sx, sx[0]), np.append(sy, sy[0])
         pts = np.stack([sx, sy], 1)
@@ -104,8 +108,9 @@ def _obstacles(rings, heights, kinds, footprints, origin_m, extent_m, metres_per
         runs_arr["outline"], runs_arr["first_edge"], runs_arr["edge_count"] = r[:, 0], r[:, 1], r[:, 2]
     verts = np.concatenate(vertices) if vertices else np.zeros((0, 2), np.int64)
     assert verts.size == 0 or (np.abs(verts).max() < 2**31)
-    return abi.Obstacles(origin_steps, metres_per_step, (cell_x0, cell_y0), (cells_x, cells_y), cell_first_run,
-                         runs_arr, outlines, verts.astype(np.int32))
+    return abi.Obstacles(origin_steps, mpx, (cell_x0, cell_y0), (cells_x, cells_y), cell_first_run,
+                         runs_arr, outlines, verts.astype(np.int32), metres_per_step_y=0.0 if metres_per_step_y is None
+                         else mpy)
 
 
 def _inside(ring, x, y):
@@ -218,6 +223,112 @@ def road_scene(seed=1, receivers=256, spacing=10.0, lines=200, per_line=10, buil
     _own_footprints(points, rings, footprints, spacing, (spacing / 2, spacing / 2), (receivers, receivers))
     scene = abi.Scene(terrain, obstacles, weather)
     return SyntheticScene("road", scene, sources, points, (receivers, receivers), float("inf"), rings)
+
+
+# The geographic frame (tiles/src/geo.rs): z12 Web Mercator units, WGS84 metres per unit at the
+# frame's origin, the one arc-second terrain lattice and the world weather table.
+TILES_PER_AXIS = 4096
+WGS84_A_M = 6_378_137.0
+WGS84_E2 = 6.694_379_990_141_316e-3
+TERRAIN_NODES_PER_DEGREE = 3600
+STEPS_PER_TILE = 32_768.0
+
+
+def mercator(lat_deg, lon_deg):
+    """geo::Mercator::from_degrees: (x, y) in z12 tile units, y growing south."""
+    n = float(TILES_PER_AXIS)
+    lon = (np.asarray(lon_deg, np.float64) + 180.0) % 360.0 - 180.0
+    phi = np.radians(np.asarray(lat_deg, np.float64))
+    return (lon + 180.0) / 360.0 * n, (1.0 - np.arcsinh(np.tan(phi)) / math.pi) / 2.0 * n
+
+
+def mercator_degrees(x, y):
+    """geo::Mercator::to_degrees: (latitude, longitude)."""
+    n = float(TILES_PER_AXIS)
+    lat = np.degrees(np.arctan(np.sinh(math.pi * (1.0 - 2.0 * np.asarray(y, np.float64) / n))))
+    return lat, np.asarray(x, np.float64) / n * 360.0 - 180.0
+
+
+def local_frame(origin_x, origin_y):
+    """geo::LocalFrame::at: east and north metres per Mercator unit at the origin (WGS84)."""
+    lat, _ = mercator_degrees(origin_x, origin_y)
+    phi = math.radians(float(lat))
+    w = 1.0 - WGS84_E2 * math.sin(phi) ** 2
+    radians_per_unit = 2.0 * math.pi / TILES_PER_AXIS
+    east = WGS84_A_M / math.sqrt(w) * math.cos(phi) * radians_per_unit
+    north = WGS84_A_M * (1.0 - WGS84_E2) / w ** 1.5 * math.cos(phi) * radians_per_unit
+    return east, north
+
+
+def geo_scene(seed=21, lat_deg=50.0875, lon_deg=14.4214, receivers=64, spacing=10.0, lines=30, per_line=10,
+              buildings=150, relief_m=40.0, npd=2):
+    """A square in the geographic frame: a frame at (lat, lon), the one arc-second terrain lattice
+    about the painted area (rows at their Mercator latitudes: ``row_y_m``; columns evenly:
+    ``column_m``), smooth terrain of `relief_m`, buildings and road points as in the road scene, the
+    obstacle steps' east and north metres from the frame (STEPS_PER_TILE a unit) and the world
+    weather table (`npd` nodes a degree) read by latitude and longitude."""
+    rng = np.random.default_rng(seed)
+    ox, oy = mercator(lat_deg, lon_deg)
+    east_m, north_m = local_frame(ox, oy)
+    extent = receivers * spacing
+    margin = 400.0
+    # The lattice's rows and columns about [-margin, extent + margin] in both axes.
+    lat_lo, _ = mercator_degrees(ox, oy + margin / north_m)
+    lat_hi, _ = mercator_degrees(ox, oy - (extent + margin) / north_m)
+    _, lon_lo = mercator_degrees(ox - margin / east_m, oy)
+    _, lon_hi = mercator_degrees(ox + (extent + margin) / east_m, oy)
+    row_lo, row_hi = int(math.floor(lat_lo * TERRAIN_NODES_PER_DEGREE)), int(math.ceil(lat_hi * TERRAIN_NODES_PER_DEGREE))
+    col_lo, col_hi = int(math.floor(lon_lo * TERRAIN_NODES_PER_DEGREE)), int(math.ceil(lon_hi * TERRAIN_NODES_PER_DEGREE))
+    row_lat = np.arange(row_lo, row_hi + 1) / TERRAIN_NODES_PER_DEGREE
+    _, row_y = mercator(row_lat, np.zeros_like(row_lat))
+    row_y_m = (oy - row_y) * north_m
+    col_lon = np.arange(col_lo, col_hi + 1) / TERRAIN_NODES_PER_DEGREE
+    col_x, _ = mercator(np.zeros_like(col_lon), col_lon)
+    col_east = (col_x - ox) * east_m
+    column_m = float(col_east[1] - col_east[0])
+    ee, nn = np.meshgrid(col_east, row_y_m)
+    height = 200.0 + relief_m * (0.5 * np.sin(ee / 310.0) * np.cos(nn / 260.0) + 0.3 * np.sin((ee + 2 * nn) / 170.0))
+    height += rng.normal(0.0, 0.3, height.shape)
+    impervious = rng.choice(np.array([100, 50, 0]), size=height.shape)
+    terrain = abi.Terrain(float(col_east[0]), float(row_y_m[0]), float(np.mean(np.diff(row_y_m))),
+                          abi.Terrain.pack(height, impervious), column_m=column_m, row_y_m=row_y_m)
+    # Buildings: rotated rectangles; obstacle steps of the frame (anisotropic).
+    centres = rng.uniform(0, extent, (buildings, 2))
+    sides = rng.uniform(5.0, 40.0, (buildings, 2))
+    angles = rng.uniform(0, math.pi, buildings)
+    heights = rng.uniform(3.0, 30.0, buildings)
+    rings = []
+    for (cx, cy), (a, b), th in zip(centres, sides, angles):
+        c, s_ = math.cos(th), math.sin(th)
+        corners = [(-a / 2, -b / 2), (a / 2, -b / 2), (a / 2, b / 2), (-a / 2, b / 2)]
+        rings.append([(cx + c * u - s_ * v, cy + s_ * u + c * v) for u, v in corners])
+    footprints = np.arange(1, buildings + 1, dtype=np.uint64)
+    obstacles = _obstacles(rings, heights, [abi.KIND_BUILDING] * buildings, footprints, (0.0, 0.0), (extent, extent),
+                           metres_per_step=east_m / STEPS_PER_TILE, metres_per_step_y=north_m / STEPS_PER_TILE)
+    # The world weather table: favourable percentages and absorptions varying node by node.
+    rows_w, cols_w = 180 * npd + 1, 360 * npd
+    nodes = np.zeros((rows_w, cols_w, abi.WEATHER_NODE_BYTES), np.uint8)
+    rr, cc = np.meshgrid(np.arange(rows_w), np.arange(cols_w), indexing="ij")
+    for p in range(3):
+        for k in range(16):
+            nodes[:, :, p * 16 + k] = (20 + (rr * 7 + cc * 13 + p * 11 + k * 5) % 61).astype(np.uint8)
+    alpha = np.round(ALPHA_15C_70 * 100.0)[None, None, :] * (1.0 + 0.02 * ((rr + cc) % 7))[:, :, None]
+    nodes[:, :, 48:] = alpha.astype("<u2").view(np.uint8).reshape(rows_w, cols_w, 16)
+    weather = abi.Weather(0.0, 0.0, 1.0, nodes, frame=(ox, oy, east_m, north_m), nodes_per_degree=npd)
+    xy = _polyline_points(rng, lines, per_line, (extent, extent), (0.0, 0.0))
+    sources = np.zeros(len(xy), abi.SOURCE_DTYPE)
+    sources["x_m"], sources["y_m"] = xy[:, 0], xy[:, 1]
+    sources["height_m"] = 0.05
+    sources["exclusion_radius_m"] = 2.0
+    sources["ground_percent"] = abi.GROUND_FROM_TERRAIN
+    levels = rng.normal(80.0, 6.0, (len(xy), 8)) + A_WEIGHTING_DB
+    sources["energy"] = np.repeat((10.0 ** (levels / 10.0))[:, None, :], 3, axis=1)
+    points = receiver_grid(receivers, receivers, spacing, (spacing / 2, spacing / 2))
+    _own_footprints(points, rings, footprints, spacing, (spacing / 2, spacing / 2), (receivers, receivers))
+    scene = abi.Scene(terrain, obstacles, weather)
+    sc = SyntheticScene("geo", scene, sources, points, (receivers, receivers), float("inf"), rings)
+    sc.frame = (ox, oy, east_m, north_m)
+    return sc
 
 
 def _diamond_square(rng, power, relief):
@@ -543,4 +654,4 @@ def with_fine_terrain(terrain, seed=13, share=0.5, amplitude_m=4.0):
     blocks = (rng.uniform(size=((rows - 2) // abi.FINE_BLOCK_CELLS + 1, (columns - 2) // abi.FINE_BLOCK_CELLS + 1))
               < share).astype(np.uint8)
     return abi.Terrain(terrain.origin_x_m, terrain.origin_y_m, terrain.spacing_m, terrain.nodes, terrain.canopy_m,
-                       fine_detail=detail, fine_blocks=blocks)
+                       fine_detail=detail, fine_blocks=blocks, column_m=terrain.column_m, row_y_m=terrain.row_y_m)
