@@ -11,6 +11,9 @@ loudness kernel (pixel_loudness_kernel) over `--pixels` synthetic packed pixels,
 `flights` sums `--flight-boxes` synthetic aircraft boxes over 20 km at `--flight-points` points
 (every point hearing all 16 tiles: flight_sums_kernel), `horizons` builds the horizons of
 `--horizon-points` receivers of the tile-scale scene at `--scale` 0.25 (horizons_kernel), and
+`mosaic` paints the batched uneven tiles configuration at full size (`--mosaic-tiles` a side of
+256 x 256 receivers, log-normal sources and buildings) through autotuned captured plans of
+`--mosaic-batch` tiles (tiles_kernel), and
 `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
 line when it ends.
@@ -40,6 +43,8 @@ def main():
     ap.add_argument("--flight-boxes", type=int, default=20000)
     ap.add_argument("--flight-points", type=int, default=4096)
     ap.add_argument("--horizon-points", type=int, default=4096)
+    ap.add_argument("--mosaic-tiles", type=int, default=64, help="tiles a side of the batched mosaic")
+    ap.add_argument("--mosaic-batch", type=int, default=256, help="tiles a captured plan paints")
     args = ap.parse_args()
     kernels = args.kernels.split(",")
     gpu = Gpu(0)
@@ -128,6 +133,50 @@ def main():
         t = time.time()
         rec, failed = gpu.horizons(square, pts)
         print(f"horizons: {pts.size} receivers, {int((failed != 0).sum())} failed, {time.time() - t:.2f} s", flush=True)
+    if "mosaic" in kernels:
+        # The batched uneven tiles at full size: tiles of 256 x 256 receivers with log-normal
+        # source (median 500) and building (median 300) counts, each against its own sources;
+        # the autotuner picks (block_sources, groups_per_cu) on 8 sampled tiles, then captured
+        # plans of --mosaic-batch tiles (each one persistent launch in a hipGraph) paint them all.
+        from . import tiles as tiles_mod
+
+        t = time.time()
+        m = scenes.tile_mosaic_fast(tiles=(args.mosaic_tiles, args.mosaic_tiles))
+        n = m.points.size
+        table = m.tile_table()
+        print(f"mosaic: {m.n_tiles} tiles, {n} receivers, {m.sources.size} sources, "
+              f"{m.scene.obstacles.outlines.size} buildings, generated in {time.time() - t:.1f} s", flush=True)
+        square = gpu.upload(m.scene, m.sources)
+        d_points = hip.DeviceBuffer.from_array(m.points)
+        d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
+        gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)
+        hip.synchronize()
+        del d_points
+        tuned = tiles_mod.autotune(gpu, square, table, d_rec.ptr, n)
+        print(f"mosaic: autotuned on {tuned['sample_tiles']} tiles: block_sources {tuned['block_sources']}, "
+              f"groups_per_cu {tuned['groups_per_cu']}", flush=True)
+        d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+        stream = hip.Stream()
+        total_paths, total_s, counted, failed = 0, 0.0, 0, 0
+        for at in range(0, m.n_tiles, args.mosaic_batch):
+            plan = tiles_mod.TilePlan(gpu, square, table[at:at + args.mosaic_batch], d_rec.ptr, n,
+                                      tuned["block_sources"], tuned["groups_per_cu"], d_e.ptr, d_p.ptr, d_f.ptr)
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(stream)
+            plan.run(stream.handle)
+            e1.record(stream)
+            e1.synchronize()
+            s_ = e1.seconds_since(e0)
+            total_paths += plan.paths
+            total_s += s_
+            # A run clears every receiver's outputs, then paints its own tiles'.
+            counted += int(d_p.to_array(np.uint32, n).astype(np.int64).sum())
+            failed += int(d_f.to_array(np.uint32, n).astype(np.int64).sum())
+            print(f"mosaic: tiles {at}-{min(at + args.mosaic_batch, m.n_tiles) - 1}: {plan.paths} paths in "
+                  f"{s_:.2f} s ({plan.paths / s_:.3e} paths/s)", flush=True)
+            plan.close()
+        print(f"mosaic: {total_paths} paths ({counted} counted on the card, {failed} over the per-path buffers) in "
+              f"{total_s:.1f} s: {total_paths / total_s:.3e} paths/s", flush=True)
     if "copy" in kernels:
         nbytes = args.copy_mib << 20
         a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)

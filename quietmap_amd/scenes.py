@@ -561,6 +561,127 @@ def tile_mosaic(seed=4, tiles=(4, 4), receivers=256, spacing=10.0, source_median
                       np.array(first, np.uint32))
 
 
+def _obstacles_fast(rings_xy, ring_size, heights, origin_m, extent_m, metres_per_step=METRES_PER_STEP):
+    """The cell index of many closed polygons at once (as _obstacles, vectorized): `rings_xy` the
+    vertices (total, 2) in metres of rings of `ring_size` vertices each (not yet closed). Each
+    edge is listed in every cell of its step box padded by one step; an outline's consecutive
+    edges in one cell form one run."""
+    mps = metres_per_step
+    origin_steps = (10_000.0, 10_000.0 + extent_m[1] / mps + 5_000.0)
+    cs = abi.CELL_STEPS
+    cell_x0 = int(math.floor((origin_steps[0] + (origin_m[0] - 2000.0) / mps) / cs))
+    cell_y0 = int(math.floor((origin_steps[1] - (origin_m[1] + extent_m[1] + 2000.0) / mps) / cs))
+    cells_x = int(math.floor((origin_steps[0] + (origin_m[0] + extent_m[0] + 2000.0) / mps) / cs)) - cell_x0 + 1
+    cells_y = int(math.floor((origin_steps[1] - (origin_m[1] - 2000.0) / mps) / cs)) - cell_y0 + 1
+    ring_size = np.asarray(ring_size, np.int64)
+    n_out = ring_size.size
+    first_in = np.concatenate([[0], np.cumsum(ring_size)[:-1]])
+    sx = np.round(origin_steps[0] + rings_xy[:, 0] / mps).astype(np.int64)
+    sy = np.round(origin_steps[1] - rings_xy[:, 1] / mps).astype(np.int64)
+    # Closed rings: each outline's first vertex repeated after its last.
+    closed = ring_size + 1
+    first_out = np.concatenate([[0], np.cumsum(closed)[:-1]])
+    total = int(closed.sum())
+    vx, vy = np.empty(total, np.int64), np.empty(total, np.int64)
+    owner_in = np.repeat(np.arange(n_out), ring_size)
+    pos_in = np.arange(sx.size) - first_in[owner_in]
+    vx[first_out[owner_in] + pos_in], vy[first_out[owner_in] + pos_in] = sx, sy
+    vx[first_out + ring_size], vy[first_out + ring_size] = sx[first_in], sy[first_in]
+    # Edges: outline o, edge e from vertex e to e + 1.
+    e_owner = np.repeat(np.arange(n_out), ring_size)
+    e_index = np.arange(e_owner.size) - np.concatenate([[0], np.cumsum(ring_size)[:-1]])[e_owner]
+    a = first_out[e_owner] + e_index
+    ax, ay, bx, by = vx[a], vy[a], vx[a + 1], vy[a + 1]
+    c0x = (np.minimum(ax, bx) - 1) // cs - cell_x0
+    c0y = (np.minimum(ay, by) - 1) // cs - cell_y0
+    c1x = (np.maximum(ax, bx) + 1) // cs - cell_x0
+    c1y = (np.maximum(ay, by) + 1) // cs - cell_y0
+    wx, wy = c1x - c0x + 1, c1y - c0y + 1
+    count = wx * wy
+    k = np.repeat(np.arange(e_owner.size), count)
+    local = np.arange(k.size) - np.concatenate([[0], np.cumsum(count)[:-1]])[k]
+    cell = (c0y[k] + local // wx[k]) * cells_x + (c0x[k] + local % wx[k])
+    order = np.lexsort((e_index[k], e_owner[k], cell))
+    cell, o, e = cell[order], e_owner[k][order], e_index[k][order]
+    new_run = np.ones(cell.size, bool)
+    new_run[1:] = (cell[1:] != cell[:-1]) | (o[1:] != o[:-1]) | (e[1:] != e[:-1] + 1)
+    starts = np.flatnonzero(new_run)
+    lengths = np.diff(np.append(starts, cell.size))
+    runs = np.zeros(starts.size, abi.RUN_DTYPE)
+    runs["outline"], runs["first_edge"], runs["edge_count"] = o[starts], e[starts], lengths
+    run_cell = cell[starts]
+    cell_first_run = np.searchsorted(run_cell, np.arange(cells_x * cells_y + 1)).astype(np.uint32)
+    outlines = np.zeros(n_out, abi.OUTLINE_DTYPE)
+    outlines["footprint_id"] = np.arange(1, n_out + 1, dtype=np.uint64)
+    outlines["first_vertex"] = first_out
+    outlines["vertex_count"] = closed
+    outlines["height_dm"] = np.round(np.asarray(heights) * 10.0).astype(np.uint16)
+    outlines["kind"] = abi.KIND_BUILDING
+    verts = np.stack([vx, vy], 1).astype(np.int32)
+    return abi.Obstacles(origin_steps, mps, (cell_x0, cell_y0), (cells_x, cells_y), cell_first_run, runs, outlines,
+                         verts)
+
+
+def tile_mosaic_fast(seed=4, tiles=(64, 64), receivers=256, spacing=10.0, source_median=500, building_median=300,
+                     sigma=1.0, source_clip=(10, 50_000), building_clip=(1, 50_000)):
+    """Configuration "batched uneven tiles" at its full size, generated vectorized: as
+    tile_mosaic (log-normal per-tile source and building counts, one diamond-square DEM at 30 m)
+    with sources at uniform places in their tile and receivers standing outdoors everywhere (no
+    receiver takes a building as its own footprint)."""
+    rng = np.random.default_rng(seed)
+    tx, ty = tiles
+    side = receivers * spacing
+    extent = (tx * side, ty * side)
+    margin = 600.0
+    nodes_needed = int(math.ceil((max(extent) + 2 * margin) / 30.0)) + 1
+    power = max(int(math.ceil(math.log2(max(nodes_needed - 1, 2)))), 2)
+    dem = _diamond_square(rng, power, 200.0)[:nodes_needed, :nodes_needed] + 100.0
+    impervious = np.clip(rng.normal(40, 30, dem.shape), 0, 100).astype(np.uint32)
+    terrain = abi.Terrain(-margin, -margin, 30.0, abi.Terrain.pack(dem, impervious))
+    del dem, impervious
+    n_tiles = tx * ty
+    n_src = np.clip(np.round(source_median * np.exp(sigma * rng.normal(size=n_tiles))), *source_clip).astype(np.int64)
+    n_bld = np.clip(np.round(building_median * np.exp(sigma * rng.normal(size=n_tiles))), *building_clip).astype(np.int64)
+    tile_x, tile_y = (np.arange(n_tiles) % tx) * side, (np.arange(n_tiles) // tx) * side
+    # Buildings: 4-12-gons of radius 5-20 m about uniform centres in their tile.
+    owner = np.repeat(np.arange(n_tiles), n_bld)
+    nb = owner.size
+    centre = rng.uniform(0, side, (nb, 2)) + np.stack([tile_x[owner], tile_y[owner]], 1)
+    k = rng.integers(4, 13, nb)
+    b = np.repeat(np.arange(nb), k)
+    j = np.arange(b.size) - np.concatenate([[0], np.cumsum(k)[:-1]])[b]
+    # Sorted angles per polygon: evenly spaced plus jitter within each sector.
+    ang = (j + rng.uniform(0.1, 0.9, b.size)) * (2 * math.pi / k[b])
+    rad = rng.uniform(5.0, 20.0, nb)[b] * rng.uniform(0.6, 1.0, b.size)
+    xy = np.stack([centre[b, 0] + rad * np.cos(ang), centre[b, 1] + rad * np.sin(ang)], 1)
+    obstacles = _obstacles_fast(xy, k, rng.uniform(3.0, 60.0, nb), (0.0, 0.0), extent)
+    del xy, b, j, ang, rad
+    weather = _weather((-margin, -margin), (extent[0] + 2 * margin, extent[1] + 2 * margin),
+                       lambda r, c: np.full((3, 16), 50))
+    owner_s = np.repeat(np.arange(n_tiles), n_src)
+    src = np.zeros(owner_s.size, abi.SOURCE_DTYPE)
+    src["x_m"] = rng.uniform(0, side, owner_s.size) + tile_x[owner_s]
+    src["y_m"] = rng.uniform(0, side, owner_s.size) + tile_y[owner_s]
+    rail = rng.uniform(0, 1, owner_s.size) < 0.3
+    src["height_m"] = np.where(rail, 0.5, 0.05)
+    src["layer"] = np.where(rail, 1, 0)
+    src["exclusion_radius_m"] = 2.0
+    src["ground_percent"] = abi.GROUND_FROM_TERRAIN
+    levels = rng.normal(85.0, 8.0, (owner_s.size, 8)) + A_WEIGHTING_DB
+    src["energy"] = 10.0 ** ((levels[:, None, :] + np.array([0.0, -2.0, -7.0])[None, :, None]) / 10.0)
+    first = np.concatenate([[0], np.cumsum(n_src)]).astype(np.uint32)
+    # Receivers tile by tile (each tile's 256 x 256 block contiguous).
+    ii, jj = np.meshgrid(np.arange(receivers), np.arange(receivers))
+    local = np.stack([(ii.ravel() + 0.5) * spacing, (jj.ravel() + 0.5) * spacing], 1)
+    pts = np.zeros(n_tiles * receivers * receivers, abi.POINT_DTYPE)
+    per = receivers * receivers
+    for t in range(n_tiles):
+        pts["x_m"][t * per:(t + 1) * per] = local[:, 0] + tile_x[t]
+        pts["y_m"][t * per:(t + 1) * per] = local[:, 1] + tile_y[t]
+    scene = abi.Scene(terrain, obstacles, weather)
+    return TileMosaic(scene, src, pts, (receivers, receivers), (tx, ty), first)
+
+
 def line_pieces(rng, count_lines, extent, origin, piece_m=250.0, height_m=0.05, level_mean=75.0, level_sd=6.0):
     """Random polylines (2-5 vertices over the extent) cut into straight pieces of at most
     `piece_m`, with an emission per metre of N(level_mean, level_sd) dB per band + A-weighting."""

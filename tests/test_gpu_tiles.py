@@ -98,3 +98,66 @@ def test_autotune_picks_a_candidate(gpu, mosaic):
     assert (res["block_sources"], res["groups_per_cu"]) in ((16, 1), (64, 2))
     assert all(t["paths_per_s"] > 0 for t in res["timings"])
     square.close()
+
+
+def test_hundreds_of_tiles_at_the_configured_counts(gpu):
+    """256 tiles with the configuration's log-normal counts (sources median 500, buildings median
+    300, clipped 10..50k) in one autotuned captured plan, on tiles of the configuration's side
+    (2.4 km) with 24 x 24 receivers at 100 m; 160 sampled receivers against the oracle (each
+    against its own tile's sources), paths and failures alike."""
+    from quietmap_amd import hip, scenes
+    from quietmap_amd import tiles as T
+    from quietmap_amd.painter import RECEIVER_BYTES, lden_db
+    from quietmap_amd.tiles import TilePlan
+
+    print("case: tiles 16x16 of 24x24 at median 500 sources", flush=True)
+    m = scenes.tile_mosaic_fast(seed=9, tiles=(16, 16), receivers=24, spacing=100.0)
+    n = m.points.size
+    square = gpu.upload(m.scene, m.sources)
+    d_points = hip.DeviceBuffer.from_array(m.points)
+    d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
+    gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)
+    tiles = m.tile_table()
+    counts = tiles["source_count"]
+    print(f"tiles {tiles.size}: sources median {np.median(counts):.0f}, min {counts.min()}, max {counts.max()}",
+          flush=True)
+    assert 350 <= np.median(counts) <= 700 and counts.max() > 5 * np.median(counts)
+    tuned = T.autotune(gpu, square, tiles, d_rec.ptr, n, sample=6, repeats=1)
+    d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+    plan = TilePlan(gpu, square, tiles, d_rec.ptr, n, tuned["block_sources"], tuned["groups_per_cu"], d_e.ptr, d_p.ptr,
+                    d_f.ptr)
+    plan.run()
+    hip.synchronize()
+    energy = d_e.to_array(np.float64, (n, 3))
+    paths = d_p.to_array(np.uint32, n)
+    fails = d_f.to_array(np.uint32, n)
+    assert int(paths.astype(np.int64).sum()) == plan.paths
+    rng = np.random.default_rng(10)
+    pick = rng.choice(n, 160, replace=False)
+    tile_of = pick // (24 * 24)
+    pp, ps = [], []
+    for j, (i, t) in enumerate(zip(pick, tile_of)):
+        s = np.arange(tiles["source_first"][t], tiles["source_first"][t] + tiles["source_count"][t], dtype=np.uint32)
+        pp.append(np.full(s.size, i, np.uint32))
+        ps.append(s)
+    pp, ps = np.concatenate(pp), np.concatenate(ps)
+    ref = oracle.evaluate_pairs(m.scene, m.points, m.sources, pp, ps, point_bands=False)
+    want = np.zeros((n, 3))
+    np.add.at(want, pp, ref["sums"][:, :3])
+    assert np.array_equal(paths[pick].astype(np.int64), np.bincount(pp, minlength=n)[pick])
+    # The card counts a path it cannot hold (more crossings, hull points or roofs than its
+    # buffers: the densest tiles) as failed, where the oracle has no bound; every receiver the
+    # card painted whole agrees with the oracle, which fails none of its paths.
+    want_fails = np.bincount(pp, weights=(ref["failed"] != 0), minlength=n)[pick]
+    whole = fails[pick] == 0
+    assert np.all(want_fails[whole] == 0)
+    share = fails.astype(np.int64).sum() / plan.paths
+    err = np.abs(lden_db(energy[pick][whole]) - lden_db(want[pick][whole]))
+    assert not np.isinf(err).any()
+    print(f"tiles at the configured counts: {plan.paths} paths, {share:.2e} of them over the buffers; Lden over "
+          f"{int(whole.sum())} whole receivers max {err.max():.5f} dB (block {tuned['block_sources']}, groups "
+          f"{tuned['groups_per_cu']})", flush=True)
+    assert share < 2e-2 and whole.mean() > 0.8
+    assert err.max() <= 0.05
+    plan.close()
+    square.close()
