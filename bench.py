@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-loudness", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--band-sources", type=int, default=200, help="sources whose paths measure the surrogate's band error")
     ap.add_argument("--no-flights", action="store_true")
     ap.add_argument("--no-doc29", action="store_true")
     ap.add_argument("--doc29-scale", type=float, default=1.0, help="side of the Doc 29 grid relative to 2048")
@@ -168,10 +169,29 @@ def main():
         lden_sur = lden_db(d_energy_s.to_array(np.float64, (n, 3)))
         err_s = np.abs(lden_sur - lden_exact)
         err_s = err_s[np.isfinite(err_s)]
+        # Per-band error: the network's A_ground + A_dif per band and state against the exact
+        # kernel's on in-envelope paths of this scene (sources and receivers it was not trained on).
+        samples, (spr, _) = surrogate.generate_samples(gpu, sc, args.band_sources, seed=2, stride=16)
+        rows = samples.rows
+        d_out = hip.DeviceBuffer(max(rows, 1) * surrogate.OUTPUTS * 4)
+        net.infer_device(samples.inputs.ptr, rows, d_out.ptr)
+        hip.synchronize()
+        band_err = np.abs(d_out.to_array(np.float32, (rows, surrogate.OUTPUTS)).astype(np.float64) -
+                          samples.targets.to_array(np.float32, (rows, surrogate.OUTPUTS)).astype(np.float64))
+        del samples, d_out
         sur = {"value": s_paths / s_time, "unit": "paths/s", "ms_per_step": s_time * 1e3,
                "speedup_vs_exact": (s_paths / s_time) / value, "network_paths_fraction": learned / max(s_paths, 1),
                "lden_vs_exact_db": {"mean": float(err_s.mean()), "p99": float(np.quantile(err_s, 0.99)),
                                     "p999": float(np.quantile(err_s, 0.999)), "max": float(err_s.max())},
+               "band_attenuation_vs_exact_db": {
+                   "paths": int(spr.size), "rows": int(rows), "mean": float(band_err.mean()),
+                   "p99": float(np.quantile(band_err, 0.99)), "p999": float(np.quantile(band_err, 0.999)),
+                   "max": float(band_err.max()),
+                   "within_0_5_db": float((band_err <= 0.5).mean()),
+                   "per_band_p99": [float(np.quantile(band_err[:, b], 0.99)) for b in range(surrogate.OUTPUTS)]},
+               "targets_met": {"speedup_3x": bool((s_paths / s_time) / value >= 3.0),
+                               "lden_0_2_db": bool(err_s.max() <= 0.2),
+                               "band_0_5_db": bool(band_err.max() <= 0.5)},
                "mlp_flop_per_inference": 417792}
         net.close()
         print(f"[bench] surrogate {sur['value']:.3e} paths/s ({sur['speedup_vs_exact']:.2f}x exact), Lden vs exact "

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64) void horizons_kernel(Scene sc, const qm_receive
         for (int p = 0; p < PIECES; ++p) {
             const double from = p * piece;
             int count = 0;
-            failed |= find_crossings(sc.obstacles, x + east * from, y + north * from, x + east * (from + piece),
+            failed |= find_crossings(sc, x + east * from, y + north * from, x + east * (from + piece),
                                      y + north * (from + piece), cross, count);
             for (int c = 0; c < count; ++c) {
                 const Crossing& k = cross[c];

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
     const int wave = lane >> 6, l = lane & 63;
     const int lx = (wave & 1) * 8 + (l & 7), ly = (wave >> 1) * 8 + (l >> 3);
     const int xcd = blockIdx.x % XCDS;
+    phase_start();
     for (;;) {
         if (lane == 0) s_tile = next_tile(g.counters, g.n_tiles, xcd);
         __syncthreads();
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
             g.failures[index] = fails;
         }
     }
+    phase_flush();
 }
 
 __global__ __launch_bounds__(256) void crossings_kernel(Scene sc, const double* segments, uint32_t n, uint32_t capacity,
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(256) void crossings_kernel(Scene sc, const double* 
     if (i >= n) return;
     Crossing c[CROSS_CAP];
     int k = 0;
-    failed[i] = find_crossings(sc.obstacles, segments[4 * i], segments[4 * i + 1], segments[4 * i + 2],
+    failed[i] = find_crossings(sc, segments[4 * i], segments[4 * i + 1], segments[4 * i + 2],
                                segments[4 * i + 3], c, k);
     count[i] = (uint32_t)k;
     for (int j = 0; j < k && j < (int)capacity; ++j) {
@@ -235,6 +237,8 @@ BandTable make_band_table() {
         t.wavenumber[b] = (float)(2.0 * 3.141592653589793 * F[b] / 340.0);
         t.foliage_short_db[b] = (float)SHORT[b];
         t.foliage_db_per_m[b] = (float)PER_M[b];
+        t.inv_wavelength[b] = (float)(F[b] / 340.0);
+        t.inv_wavenumber[b] = (float)(340.0 / (2.0 * 3.141592653589793 * F[b]));
     }
     return t;
 }
@@ -473,6 +477,22 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     sq->scene.obstacles.runs = d_runs;
     sq->scene.obstacles.outlines = d_outlines;
     sq->scene.obstacles.vertices = d_vertices;
+    {
+        // Occupancy words: a bit per cell holding runs, 8 x 8 cells a word.
+        const uint32_t bx = (obstacles->cells_x + 7) / 8, by = (obstacles->cells_y + 7) / 8;
+        std::vector<uint64_t> words((size_t)bx * by, 0);
+        for (uint32_t y = 0; y < obstacles->cells_y; ++y)
+            for (uint32_t x = 0; x < obstacles->cells_x; ++x) {
+                const size_t c = (size_t)y * obstacles->cells_x + x;
+                if (obstacles->cell_first_run[c + 1] > obstacles->cell_first_run[c])
+                    words[(size_t)(y >> 3) * bx + (x >> 3)] |= 1ull << (8 * (y & 7) + (x & 7));
+            }
+        uint64_t* d_words = nullptr;
+        if (!words.empty() && upload(sq, words.data(), words.size(), &d_words, err, errlen)) return cleanup(1);
+        sq->scene.cell_blocks = d_words;
+        sq->scene.blocks_x = bx;
+        sq->scene.blocks_y = by;
+    }
     uint8_t* d_weather;
     if (upload(sq, weather->nodes, (size_t)weather->rows * weather->columns * QM_WEATHER_NODE_BYTES, &d_weather, err,
                errlen))
@@ -754,5 +774,20 @@ int qm_crossings(qm_gpu* gpu, const qm_square* square, const double* segments, s
     QM_TRY(hipMemcpy(failed, d_failed.p, sizeof(uint32_t) * n_segments, hipMemcpyDeviceToHost));
     return 0;
 }
+
+#ifdef QM_PHASE_CLOCKS
+// The instrumented build only (not part of the ABI in include/): the phase clocks' totals of this
+// translation unit's path kernels (qm::PathPhase order), cleared afterwards when `reset`.
+int qm_phase_clocks(unsigned long long* out, int reset, char* err, size_t errlen) {
+    if (!out) return fail(err, errlen, "out must not be null");
+    QM_TRY(hipDeviceSynchronize());
+    QM_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(qm::qm_phase_cycles), sizeof(unsigned long long) * qm::PHASE_COUNT));
+    if (reset) {
+        const unsigned long long zero[qm::PHASE_COUNT] = {};
+        QM_TRY(hipMemcpyToSymbol(HIP_SYMBOL(qm::qm_phase_cycles), zero, sizeof(zero)));
+    }
+    return 0;
+}
+#endif
 
 }  // extern "C"

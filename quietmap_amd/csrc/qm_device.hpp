@@ -43,6 +43,8 @@ struct BandTable {
     float wavenumber[BANDS];
     float foliage_short_db[BANDS];
     float foliage_db_per_m[BANDS];
+    float inv_wavelength[BANDS];  // 1 / wavelength and 1 / wavenumber, so the band terms multiply
+    float inv_wavenumber[BANDS];
 };
 
 QM_DEV float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }

@@ -7,6 +7,56 @@
 
 namespace qm {
 
+// Phase clocks, compiled only into the instrumented build (QM_PHASE_CLOCKS, libqmpaint_phases.so)
+// that the profiling workload uses to split a path's time: each mark adds the shader clock cycles
+// since the wave's previous mark to the phase that mark ends (the wave's first active lane keeps
+// the wave's totals in LDS), so the phases partition every wave's time; a kernel that calls
+// phase_start and phase_flush adds its waves' totals to the device's. Each translation unit keeps
+// its own.
+enum PathPhase {
+    PHASE_OUTSIDE,     // the caller's loop (tile queue, source loop, sums)
+    PHASE_CROSSINGS,   // obstacle crossings, own and containing footprints
+    PHASE_SETUP,       // the ends' ground, wall tops' terrain, roofs, the states' lines
+    PHASE_VERTEX,      // the lattice walk's next vertex and its terrain nodes
+    PHASE_PROFILE,     // roofs and the vertex into the merged profile, wall tops' reads, foliage
+    PHASE_CREST,       // crest thinning, the tops' merge ahead of each candidate
+    PHASE_CANDIDATES,  // both states' streams over the candidates
+    PHASE_TAIL,        // the last tops, the direct path's plane
+    PHASE_BOUNDARY,    // both states' boundary terms
+    PHASE_MIX,         // foliage, air absorption, the period mix
+    PHASE_COUNT
+};
+#ifdef QM_PHASE_CLOCKS
+static __device__ unsigned long long qm_phase_cycles[PHASE_COUNT];
+static __shared__ unsigned long long qm_phase_last[QM_PATH_GROUP / 64];
+static __shared__ unsigned long long qm_phase_acc[QM_PATH_GROUP / 64][PHASE_COUNT];
+QM_DEV bool phase_lane() { return (int)(threadIdx.x & 63) == __builtin_ctzll(__builtin_amdgcn_read_exec()); }
+// At a path kernel's start: the wave's totals cleared and its clock taken.
+QM_DEV void phase_start() {
+    if (phase_lane()) {
+        for (int k = 0; k < PHASE_COUNT; ++k) qm_phase_acc[threadIdx.x >> 6][k] = 0;
+        qm_phase_last[threadIdx.x >> 6] = __builtin_readcyclecounter();
+    }
+}
+QM_DEV void phase_mark(PathPhase k) {
+    const unsigned long long now = __builtin_readcyclecounter();
+    if (phase_lane()) {
+        qm_phase_acc[threadIdx.x >> 6][k] += now - qm_phase_last[threadIdx.x >> 6];
+        qm_phase_last[threadIdx.x >> 6] = now;
+    }
+}
+// At the kernel's end: the wave's totals into the device's.
+QM_DEV void phase_flush() {
+    phase_mark(PHASE_OUTSIDE);
+    if (phase_lane())
+        for (int k = 0; k < PHASE_COUNT; ++k) atomicAdd(&qm_phase_cycles[k], qm_phase_acc[threadIdx.x >> 6][k]);
+}
+#else
+QM_DEV void phase_start() {}
+QM_DEV void phase_mark(PathPhase) {}
+QM_DEV void phase_flush() {}
+#endif
+
 // The per-path buffers (private to the work-item).
 struct PathScratch {
     Crossing cross[CROSS_CAP];
@@ -43,9 +93,11 @@ QM_DEV float favourable_at(const float* row, float azimuth) {
 
 QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
     if (!(c.x > 0.0f && c.x < length)) return;
+    phase_mark(PHASE_CREST);
     c.id = -1;
     state_candidate(lane_states(), w.hull, 0, w.log, S, R, c, failed);
     state_candidate(lane_states(), w.hull, 1, w.log, S, R, c, failed);
+    phase_mark(PHASE_CANDIDATES);
 }
 
 // The x of the first top at or after index i not dropped (+inf when none).
@@ -134,7 +186,7 @@ QM_DEV void crest_vertex(PathScratch& w, const Profile& P, CrestGroup& cg, Tops&
 // containing it (ray.rs:201-226).
 QM_DEV uint32_t path_crossings(const Scene& sc, const qm_receiver& rcv, const qm_source& src, Crossing* cross, int& n) {
     n = 0;
-    const uint32_t failed = find_crossings(sc.obstacles, src.x_m, src.y_m, rcv.x_m, rcv.y_m, cross, n);
+    const uint32_t failed = find_crossings(sc, src.x_m, src.y_m, rcv.x_m, rcv.y_m, cross, n);
     if (failed & QM_FAILED_NOT_READ) return failed;
     for (int i = 0; i < n; ++i) {
         cross[i].flags &= 3u;
@@ -287,7 +339,7 @@ QM_DEV uint32_t terrain_bracket(const qm_terrain& T, const Lattice& L, float t, 
     const float xa = tp * length, xb = tn * length;
     const float za = xa < platform ? fminf(a.z, source_ground) : a.z;
     const float zb = xb < platform ? fminf(b.z, source_ground) : b.z;
-    const float f = xb > xa ? clampf((x - xa) / (xb - xa), 0.0f, 1.0f) : 0.0f;
+    const float f = xb > xa ? clampf(fdiv(x - xa, xb - xa), 0.0f, 1.0f) : 0.0f;
     z_out = za + f * (zb - za);
     g_out = a.g + f * (b.g - a.g);
     return 0;
@@ -369,6 +421,7 @@ struct RayOptions {
 template <bool FINE = false>
 QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver& rcv, const qm_source& src,
                           PathScratch& w, PathResult& out, const RayOptions& opt = RayOptions{}) {
+    phase_mark(PHASE_OUTSIDE);
     uint32_t failed = rcv.failed;
     for (int p = 0; p < PERIODS; ++p) {
         for (int b = 0; b < BANDS; ++b) out.bands[p][b] = 0.0f;
@@ -388,6 +441,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
 
     int n = 0;
     if (opt.obstacles) failed |= path_crossings(sc, rcv, src, w.cross, n);
+    phase_mark(PHASE_CROSSINGS);
     if (failed & QM_FAILED_NOT_READ) {
         out.failed = failed;
         return;
@@ -480,7 +534,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             if (r.x1 <= covered) continue;
             if (r.x0 < covered) {
                 const float span = r.x1 - r.x0;
-                r.top0 += (covered - r.x0) / span * (r.top1 - r.top0);
+                r.top0 += fdiv(covered - r.x0, span) * (r.top1 - r.top0);
                 r.x0 = covered;
                 r.ground0 = covered_ground;
                 r.g0 = covered_g;
@@ -546,6 +600,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         stretch_to = ray.plan.to[0];
     }
     int next_column = WL.first_column, next_row = WL.first_row, south = WL.start_south;
+    phase_mark(PHASE_SETUP);
     const float inf = __builtin_inff();
     // Every line crossed is one vertex: a stretch ends within |columns| + |rise| + 2 steps of its
     // lattice over its share of the ray.
@@ -595,6 +650,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             if (t_column <= t_row) next_column += WL.column_step;
             if (failed & QM_FAILED_NO_TERRAIN) break;
         }
+        phase_mark(PHASE_VERTEX);
         const float x = t * H;
         const float z = x < platform ? fminf(v.z, source_ground) : v.z;
         foliage_sample(fol, w, x, v.z, v.forest, v.canopy, false);
@@ -613,8 +669,11 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             next_roof = r < nroofs ? w.roofs[r].x0 : inf;
         }
         if (x > covered_to) profile_emit(w, tp, n, P, x, z, v.g);
+        phase_mark(PHASE_PROFILE);
         crest_vertex(w, P, cg, tp, n, S, R, H, x, z, failed);
+        phase_mark(PHASE_CREST);
     }
+    phase_mark(PHASE_CREST);
     if (failed & (QM_FAILED_NO_TERRAIN | QM_FAILED_CAPACITY)) {
         out.failed = failed;
         return;
@@ -650,6 +709,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float path_ground = (float)(end.pg / Ld);
     const float near_source = near_source_factor(direct, path_ground, source_g);
     float att[2][BANDS];
+    phase_mark(PHASE_TAIL);
     for (int s = 0; s < 2; ++s)
         state_boundary(B, lane_states()[s], w.hull[s], w.log, s == 1, S, R, z0, Ld, end.pz, end.pxz, end.pg, direct, path_ground,
                        near_source, source_g, s0.g, failed, att[s]);
@@ -658,6 +718,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         out.att[1][b] = att[1][b];
     }
 
+    phase_mark(PHASE_BOUNDARY);
     // ray.rs:166-190: foliage, air at the slant distance, the period mix; evaluate.rs:119-152.
     const float slant = fmaxf(sqrtf(H * H + (receiver_alt - source_alt) * (receiver_alt - source_alt)), 1.0f);
     const float azimuth = atan2f((float)(ry - sy), (float)(rx - sx));
@@ -667,7 +728,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     const float hd = fmaxf((float)length_d, src.exclusion_radius_m);
     const float dz = receiver_alt - source_alt;
     const float d = fmaxf(sqrtf(hd * hd + dz * dz), 1.0f);
-    const float weight = opt.weight >= 0.0f ? opt.weight : 1.0f / (d * d * 12.589254117941673f);
+    const float weight = opt.weight >= 0.0f ? opt.weight : frcp(d * d * 12.589254117941673f);
     float fol_db[2][BANDS];
     for (int s = 0; s < 2; ++s) {
         const float depth = fol.depth[s];
@@ -687,6 +748,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             out.states[1][q] += power * ef;
         }
     }
+    phase_mark(PHASE_MIX);
     out.failed = failed;
 }
 

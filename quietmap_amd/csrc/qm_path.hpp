@@ -36,6 +36,11 @@ struct Scene {
     qm_terrain terrain;
     qm_obstacles obstacles;
     qm_weather weather;
+    // Built at upload from the obstacles' cell index: one 64-bit word per block of 8 x 8 cells
+    // (block row-major, blocks_x a row; bit 8 * (y & 7) + (x & 7) set where the cell holds runs),
+    // so the cell walk tests an empty cell from the word it holds instead of reading the index.
+    const uint64_t* cell_blocks;
+    uint32_t blocks_x, blocks_y;
 };
 
 struct GroundSample {
@@ -448,9 +453,10 @@ QM_DEV double snap_steps(double steps) {
 
 // popup/src/obstacles/crossings.rs:163-230: the cell walk, the padded piece box and the exact
 // test; every hit inserted in (t, footprint, outline, edge) order. Returns failure bits.
-QM_DEV uint32_t find_crossings(const qm_obstacles& O, double fx, double fy, double tx, double ty, Crossing* out,
+QM_DEV uint32_t find_crossings(const Scene& sc, double fx, double fy, double tx, double ty, Crossing* out,
                                int& n) {
 #pragma clang fp contract(off)
+    const qm_obstacles& O = sc.obstacles;
     n = 0;
     const double mpy = O.metres_per_step_y > 0.0 ? O.metres_per_step_y : O.metres_per_step;
     const double start[2] = {snap_steps(O.origin_steps_x + fx / O.metres_per_step),
@@ -476,11 +482,27 @@ QM_DEV uint32_t find_crossings(const qm_obstacles& O, double fx, double fy, doub
     const double pad = 1e-9 * (1.0 + fabs(delta[0]) + fabs(delta[1]));
     double t_enter = 0.0;
     uint32_t failed = 0;
+    // The occupancy word of the block the walk is in (re-read when the walk leaves the block).
+    uint32_t held_block = 0xffffffffu;
+    uint64_t held_word = 0;
     for (;;) {
         const int64_t gx = cell[0] - O.cell_x0, gy = cell[1] - O.cell_y0;
         if (gx < 0 || gy < 0 || gx >= (int64_t)O.cells_x || gy >= (int64_t)O.cells_y) return QM_FAILED_NOT_READ;
-        const size_t index = (size_t)gy * O.cells_x + (size_t)gx;
-        const uint32_t r0 = O.cell_first_run[index], r1 = O.cell_first_run[index + 1];
+        bool occupied = true;
+        if (sc.cell_blocks) {
+            const uint32_t block = (uint32_t)(gy >> 3) * sc.blocks_x + (uint32_t)(gx >> 3);
+            if (block != held_block) {
+                held_word = sc.cell_blocks[block];
+                held_block = block;
+            }
+            occupied = (held_word >> (8 * (gy & 7) + (gx & 7))) & 1ull;
+        }
+        uint32_t r0 = 0, r1 = 0;
+        if (occupied) {
+            const size_t index = (size_t)gy * O.cells_x + (size_t)gx;
+            r0 = O.cell_first_run[index];
+            r1 = O.cell_first_run[index + 1];
+        }
         if (r1 > r0) {
             const double t_exit = fmin(fmin(t_max[0], t_max[1]), 1.0);
             const double te = clampd(t_enter, 0.0, 1.0), tx2 = clampd(t_exit, 0.0, 1.0);
@@ -745,11 +767,12 @@ QM_DEV Plane plane_over(double a, double b, double z0, double z_at_a, double pz_
                         double pxz_b) {
     const double length = b - a;
     if (length < 1e-3) return Plane{0.0, z_at_a};
+    const double inv = 1.0 / length;
     const double dz = pz_b - pz_a;
     const double iz = dz - (z_at_a - z0) * length;
     const double ixz = (pxz_b - pxz_a) - a * dz - (z_at_a - z0) * 0.5 * length * length;
-    const double slope = 12.0 * (ixz - 0.5 * length * iz) / (length * length * length);
-    const double intercept_relative = iz / length - 0.5 * slope * length;
+    const double slope = 12.0 * (ixz - 0.5 * length * iz) * (inv * inv * inv);
+    const double intercept_relative = iz * inv - 0.5 * slope * length;
     return Plane{slope, z_at_a + intercept_relative - slope * a};
 }
 
@@ -762,16 +785,16 @@ struct Geometry {
 
 // mean_plane.rs:109-120
 QM_DEV Geometry geometry_over(Plane p, P2d from, P2d to) {
-    const double norm = sqrt(1.0 + p.slope * p.slope);
-    const double hs = (from.z - (p.slope * from.x + p.intercept)) / norm;
-    const double hr = (to.z - (p.slope * to.x + p.intercept)) / norm;
+    const double inv_norm = 1.0 / sqrt(1.0 + p.slope * p.slope);
+    const double hs = (from.z - (p.slope * from.x + p.intercept)) * inv_norm;
+    const double hr = (to.z - (p.slope * to.x + p.intercept)) * inv_norm;
     Geometry g;
     g.plane = p;
     g.hs = (float)fmax(hs, 0.0);
     g.hr = (float)fmax(hr, 0.0);
     g.s_below = hs < 0.0;
     g.r_below = hr < 0.0;
-    g.dp = (float)(fabs((to.x - from.x) + p.slope * (to.z - from.z)) / norm);
+    g.dp = (float)(fabs((to.x - from.x) + p.slope * (to.z - from.z)) * inv_norm);
     return g;
 }
 
@@ -785,58 +808,62 @@ QM_DEV P2d mirror(Plane p, P2d q) {
 // ground.rs:39-51
 QM_DEV float near_source_factor(const Geometry& g, float path_g, float source_g) {
     const float height_sum = fmaxf(g.hs + g.hr, MINIMUM_HEIGHT_SUM_M);
-    const float ratio = g.dp / (SHORT_PATH_HEIGHT_FACTOR * height_sum);
+    const float ratio = fdiv(g.dp, SHORT_PATH_HEIGHT_FACTOR * height_sum);
     return ratio <= 1.0f ? path_g * ratio + source_g * (1.0f - ratio) : path_g;
 }
 
-// ground.rs:78-89
-QM_DEV float analytic_ground_db(const BandTable& B, int band, float dp, float zs, float zr, float gw13) {
-    const float k = B.wavenumber[band];
-    const float gw26 = gw13 * gw13;
-    const float w = 0.0185f * B.f_2_5[band] * gw26 / (B.f_1_5[band] * gw26 + 1.3e3f * B.f_0_75[band] * gw13 + 1.16e6f);
+// ground.rs:78-89, with the hardware reciprocal and square root (1 ulp) and the band's 1/k from
+// the table: 4k^2/dp^2 taken as 4(k/dp)^2.
+QM_DEV float analytic_ground_db(const BandTable& B, int band, float dp, float inv_dp, float zs, float zr, float gw13,
+                                float gw26) {
+    const float w = fdiv(0.0185f * B.f_2_5[band] * gw26, B.f_1_5[band] * gw26 + 1.3e3f * B.f_0_75[band] * gw13 + 1.16e6f);
     const float wd = w * dp;
-    const float cf = dp * (1.0f + 3.0f * wd * __expf(-sqrtf(wd))) / (1.0f + wd);
-    const float root = sqrtf(2.0f * cf / k);
-    const float ck = cf / k;
+    const float cf = fdiv(dp * (1.0f + 3.0f * wd * __expf(-fsqrt(wd))), 1.0f + wd);
+    const float ck = cf * B.inv_wavenumber[band];
+    const float root = fsqrt(2.0f * ck);
     const float product = (zs * zs - root * zs + ck) * (zr * zr - root * zr + ck);
-    return -10.0f * __log10f(4.0f * k * k / (dp * dp) * product);
+    const float kd = B.wavenumber[band] * inv_dp;
+    return -10.0f * __log10f(4.0f * kd * kd * product);
 }
 
 // ground.rs:105-137 A_ground of one (sub-)path in one state, per band.
 QM_DEV void ground_bands(const BandTable& B, const Geometry& g, float path, float impedance, float floor_factor,
                          bool favourable, float out[BANDS]) {
     const float dp = fmaxf(g.dp, 1e-6f);
+    const float inv_dp = frcp(dp);
     const float zs = g.hs, zr = g.hr;
     const float gw13 = __powf(fmaxf(impedance, 0.0f), 1.3f);
+    const float gw26 = gw13 * gw13;
     if (!favourable) {
         if (path < HARD_GROUND_BELOW) {
             for (int b = 0; b < BANDS; ++b) out[b] = -3.0f;
             return;
         }
         const float floor = -3.0f * (1.0f - floor_factor);
-        for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, zs, zr, gw13), floor);
+        for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, inv_dp, zs, zr, gw13, gw26), floor);
         return;
     }
     const float height_sum0 = fmaxf(g.hs + g.hr, MINIMUM_HEIGHT_SUM_M);
     const float base = -3.0f * (1.0f - floor_factor);
     const float limit = SHORT_PATH_HEIGHT_FACTOR * height_sum0;
-    const float floor = g.dp <= limit ? base : base * (1.0f + 2.0f * (1.0f - limit / g.dp));
+    const float floor = g.dp <= limit ? base : base * (1.0f + 2.0f * (1.0f - fdiv(limit, g.dp)));
     if (path < HARD_GROUND_BELOW) {
         for (int b = 0; b < BANDS; ++b) out[b] = floor;
         return;
     }
     const float height_sum = fmaxf(zs + zr, MINIMUM_HEIGHT_SUM_M);
     const float curvature = FAVOURABLE_CURVATURE_A0_PER_M * dp * dp * 0.5f;
-    const float terrain = FAVOURABLE_TERRAIN_HEIGHT_COEFFICIENT * dp / height_sum;
-    const float rs = zs / height_sum, rr = zr / height_sum;
+    const float inv_height_sum = frcp(height_sum);
+    const float terrain = FAVOURABLE_TERRAIN_HEIGHT_COEFFICIENT * dp * inv_height_sum;
+    const float rs = zs * inv_height_sum, rr = zr * inv_height_sum;
     const float zsf = zs + curvature * (rs * rs) + terrain;
     const float zrf = zr + curvature * (rr * rr) + terrain;
-    for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, zsf, zrf, gw13), floor);
+    for (int b = 0; b < BANDS; ++b) out[b] = fmaxf(analytic_ground_db(B, b, dp, inv_dp, zsf, zrf, gw13, gw26), floor);
 }
 
 // diffraction.rs:19-30
 QM_DEV float diffraction_db(const BandTable& B, int band, float delta, float c_second) {
-    const float x = 40.0f * c_second * delta / B.wavelength[band];
+    const float x = 40.0f * c_second * delta * B.inv_wavelength[band];
     return x >= -2.0f ? 10.0f * __log10f(3.0f + x) : 0.0f;
 }
 // diffraction.rs:33-35
@@ -1004,7 +1031,7 @@ QM_DEV void segment_at(const Vertex& a, float bx, float bz, float bg, float x, d
         c.pg = a.pg;
         return;
     }
-    const float f = (x - a.x) / (bx - a.x);
+    const float f = fdiv(x - a.x, bx - a.x);
     c.z_at = a.z + f * (bz - a.z);
     c.g_at = a.g + f * (bg - a.g);
     const double u0 = a.x, u1 = x;
@@ -1102,8 +1129,8 @@ QM_DEV void state_boundary(const BandTable& B, StateStream& st, Hull& hull, cons
         if (!admitted[b]) continue;
         float c_second = 1.0f;
         if (n > 1 && span_f > MULTIPLE_DIFFRACTION_MINIMUM_SPAN_M) {
-            const float x = 5.0f * B.wavelength[b] / span_f;
-            c_second = (1.0f + x * x) / (1.0f / 3.0f + x * x);
+            const float x = 5.0f * B.wavelength[b] * frcp(span_f);
+            c_second = fdiv(1.0f + x * x, 1.0f / 3.0f + x * x);
         }
         const float direct_db = diffraction_db(B, b, d_direct, c_second);
         const float s_db = diffraction_db(B, b, d_s, c_second);

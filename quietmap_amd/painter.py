@@ -18,6 +18,9 @@ from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "libqmpaint.so")
+# The same library with the path kernels' phase clocks (quietmap_amd/build.py): profiling only.
+PHASES_LIBRARY = os.path.join(HERE, "libqmpaint_phases.so")
+PHASES = ("outside", "crossings", "setup", "vertex", "profile", "crest", "candidates", "tail", "boundary", "mix")
 
 # Every entry point include/qm_paint.h declares.
 EXPORTS = (
@@ -78,13 +81,37 @@ class PainterError(RuntimeError):
     pass
 
 
+_phase_build = False
+
+
+def use_phase_clocks():
+    """Loads the phase-clock build instead of libqmpaint.so (before anything else loads it)."""
+    global _phase_build
+    if _lib is not None and not _phase_build:
+        raise PainterError("libqmpaint is already loaded: choose the phase-clock build first")
+    _phase_build = True
+
+
+def phase_clocks(reset: bool = True) -> dict:
+    """The phase-clock build's totals of shader-clock cycles per phase of the path (waves' time,
+    summed over waves), cleared when `reset`."""
+    lib = library()
+    if not hasattr(lib, "qm_phase_clocks"):
+        raise PainterError("phase clocks need the phase-clock build (painter.use_phase_clocks())")
+    out = (C.c_ulonglong * len(PHASES))()
+    err = C.create_string_buffer(512)
+    _check(lib.qm_phase_clocks(out, int(reset), err, 512), err)
+    return dict(zip(PHASES, (int(v) for v in out)))
+
+
 def library():
     """libqmpaint, loaded once; raises when it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIBRARY):
-            raise PainterError(f"{LIBRARY} is not built: run __graft_entry__.build() (hipcc, gfx950)")
-        lib = C.CDLL(LIBRARY)
+        path = PHASES_LIBRARY if _phase_build else LIBRARY
+        if not os.path.exists(path):
+            raise PainterError(f"{path} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        lib = C.CDLL(path)
         vp, sz, u32, dbl, cp = C.c_void_p, C.c_size_t, C.c_uint32, C.c_double, C.c_char_p
         lib.qm_gpu_new.argtypes = [C.c_int, C.POINTER(vp), cp, sz]
         lib.qm_gpu_name.argtypes = [vp, cp, sz]

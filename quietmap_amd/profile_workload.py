@@ -14,6 +14,8 @@ loudness kernel (pixel_loudness_kernel) over `--pixels` synthetic packed pixels,
 `mosaic` paints the batched uneven tiles configuration at full size (`--mosaic-tiles` a side of
 256 x 256 receivers, log-normal sources and buildings) through autotuned captured plans of
 `--mosaic-batch` tiles (tiles_kernel), and
+`phases` paints the `paint` window through the phase-clock build (libqmpaint_phases.so, alone in
+its process) and prints how the waves' time splits over the phases of a path, and
 `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
 line when it ends.
@@ -47,7 +49,37 @@ def main():
     ap.add_argument("--mosaic-batch", type=int, default=256, help="tiles a captured plan paints")
     args = ap.parse_args()
     kernels = args.kernels.split(",")
+    if "phases" in kernels:
+        if kernels != ["phases"]:
+            raise SystemExit("the phases leg runs alone (it loads the phase-clock build)")
+        from . import painter
+
+        painter.use_phase_clocks()
     gpu = Gpu(0)
+    if "phases" in kernels:
+        from . import painter
+
+        sc = scenes.tile_scene(scale=args.scale)
+        points, nx, ny = sc.window(args.window)
+        n = nx * ny
+        square = gpu.upload(sc.scene, sc.sources)
+        d_points = hip.DeviceBuffer.from_array(points)
+        d_rec = hip.DeviceBuffer(n * RECEIVER_BYTES)
+        d_e, d_p, d_f = hip.DeviceBuffer(n * 24), hip.DeviceBuffer(n * 4), hip.DeviceBuffer(n * 4)
+        gpu.receivers_device(square, d_points.ptr, n, d_rec.ptr)
+        hip.synchronize()
+        painter.phase_clocks(reset=True)
+        t = time.time()
+        gpu.paint_receivers_device(square, d_rec.ptr, nx, ny, sc.cutoff_m, d_e.ptr, d_p.ptr, d_f.ptr)
+        hip.synchronize()
+        seconds = time.time() - t
+        clocks = painter.phase_clocks(reset=True)
+        paths = int(d_p.to_array(np.uint32, n).astype(np.int64).sum())
+        total = sum(clocks.values())
+        print(f"phases: {nx}x{ny} receivers, {paths} paths, {seconds:.2f} s with the clocks", flush=True)
+        for name, cycles in clocks.items():
+            print(f"phases: {name:10s} {100.0 * cycles / total:6.2f} %  {cycles * 64 / max(paths, 1):9.0f} "
+                  f"wave cycles a round of 64 paths", flush=True)
     if "paint" in kernels:
         sc = scenes.tile_scene(scale=args.scale)
         points, nx, ny = sc.window(args.window)
