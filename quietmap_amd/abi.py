@@ -60,6 +60,10 @@ LINE_PIECE_DTYPE = np.dtype([("ax_m", "<f8"), ("ay_m", "<f8"), ("bx_m", "<f8"), 
                              ("height_m", "<f4"), ("layer", "u1"), ("on_deck", "u1"), ("pad", "u1", 2),
                              ("energy_per_m", "<f4", (3, 8)), ("deck_a_m", "<f4"), ("deck_b_m", "<f4")])
 assert LINE_PIECE_DTYPE.itemsize == 144
+# qm_receiver (qm_scene.h): a point's receiver properties as the card computes them.
+RECEIVER_DTYPE = np.dtype([("x_m", "<f8"), ("y_m", "<f8"), ("own_footprint", "<u8"), ("altitude_m", "<f4"),
+                           ("failed", "<u4"), ("alpha_db_per_km", "<f4", (8,)), ("favourable", "<f4", (3, 16))])
+assert RECEIVER_DTYPE.itemsize == 256
 # qm_tile (qm_tiles.h): a tile's receivers (row-major nx x ny from receiver_first) and its sources.
 TILE_DTYPE = np.dtype([("receiver_first", "<u4"), ("nx", "<u4"), ("ny", "<u4"), ("source_first", "<u4"),
                        ("source_count", "<u4")])

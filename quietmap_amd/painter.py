@@ -325,6 +325,19 @@ class Gpu:
                                            err, 512)
         _check(rc, err)
 
+    def receivers(self, square: Square, points):
+        """The receiver properties (abi.RECEIVER_DTYPE) of host points, computed on the card."""
+        from . import hip
+
+        points = np.ascontiguousarray(points, abi.POINT_DTYPE)
+        if points.size == 0:
+            return np.zeros(0, abi.RECEIVER_DTYPE)
+        d_points = hip.DeviceBuffer.from_array(points)
+        d_receivers = hip.DeviceBuffer(points.size * RECEIVER_BYTES)
+        self.receivers_device(square, d_points.ptr, points.size, d_receivers.ptr)
+        hip.synchronize()
+        return d_receivers.to_array(abi.RECEIVER_DTYPE, points.size)
+
     def paint_receivers_device(self, square: Square, d_receivers: int, nx: int, ny: int, cutoff_m: float,
                                d_energy: int, d_paths: int, d_failures: int, stream: int = 0):
         err = C.create_string_buffer(512)
