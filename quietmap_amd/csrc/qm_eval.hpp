@@ -91,12 +91,14 @@ QM_DEV float favourable_at(const float* row, float azimuth) {
     return row[lower] + fraction * (row[upper] - row[lower]);
 }
 
-QM_DEV void candidate_out(PathScratch& w, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
+QM_DEV void candidate_out(PathScratch& w, int& log_n, P2f S, P2f R, float length, Cand& c, uint32_t& failed) {
     if (!(c.x > 0.0f && c.x < length)) return;
     phase_mark(PHASE_CREST);
+    // A candidate adds at most one log entry (both states share it): room for one.
+    if (log_n >= LOG_CAP) log_n = log_compact(w.log, lane_states(), w.hull);
     c.id = -1;
-    state_candidate(lane_states(), w.hull, 0, w.log, S, R, c, failed);
-    state_candidate(lane_states(), w.hull, 1, w.log, S, R, c, failed);
+    state_candidate(lane_states(), w.hull, 0, w.log, log_n, S, R, c, failed);
+    state_candidate(lane_states(), w.hull, 1, w.log, log_n, S, R, c, failed);
     phase_mark(PHASE_CANDIDATES);
 }
 
@@ -110,8 +112,8 @@ QM_DEV float top_x_from(const PathScratch& w, int& i, int n) {
 // a top first only when strictly nearer). Inlined: out of line, its accesses to the per-path
 // buffers went through generic pointers (flat instructions waiting on all memory traffic), which
 // cost the bench step 30 %.
-__device__ void tops_before_slow(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x,
-                                 uint32_t& failed) {
+QM_DEV void tops_before_slow(PathScratch& w, int& log_n, Tops& tp, int n, P2f S, P2f R, float length, float x,
+                             uint32_t& failed) {
     while (tp.emit_x < x && tp.emitted < tp.resolved) {
         const Crossing& c = w.cross[tp.emitted];
         Cand k;
@@ -122,13 +124,14 @@ __device__ void tops_before_slow(PathScratch& w, Tops& tp, int n, P2f S, P2f R, 
         k.pz = c.pz;
         k.pxz = c.pxz;
         k.pg = c.pg;
-        candidate_out(w, S, R, length, k, failed);
+        candidate_out(w, log_n, S, R, length, k, failed);
         ++tp.emitted;
         tp.emit_x = top_x_from(w, tp.emitted, n);
     }
 }
-QM_DEV void tops_before(PathScratch& w, Tops& tp, int n, P2f S, P2f R, float length, float x, uint32_t& failed) {
-    if (tp.emit_x < x && tp.emitted < tp.resolved) tops_before_slow(w, tp, n, S, R, length, x, failed);
+QM_DEV void tops_before(PathScratch& w, int& log_n, Tops& tp, int n, P2f S, P2f R, float length, float x,
+                        uint32_t& failed) {
+    if (tp.emit_x < x && tp.emitted < tp.resolved) tops_before_slow(w, log_n, tp, n, S, R, length, x, failed);
 }
 
 // Reads the profile for every top before x on the segment from the profile's head to the vertex
@@ -154,13 +157,13 @@ QM_DEV void profile_emit(PathScratch& w, Tops& tp, int n, Profile& P, float x, f
     P.head = extend(P.head, x, z, g, P.z0);
 }
 
-QM_DEV void crest_vertex(PathScratch& w, const Profile& P, CrestGroup& cg, Tops& tp, int n, P2f S, P2f R,
+QM_DEV void crest_vertex(PathScratch& w, int& log_n, const Profile& P, CrestGroup& cg, Tops& tp, int n, P2f S, P2f R,
                          float length, float x, float z, uint32_t& failed) {
     const bool middle = x > CREST_ENDS_M && x < length - CREST_ENDS_M;
     const int group = (int)floorf(x / CREST_SPACING_M);
     if (cg.open && (!middle || cg.group != group)) {
-        tops_before(w, tp, n, S, R, length, cg.best.x, failed);
-        candidate_out(w, S, R, length, cg.best, failed);
+        tops_before(w, log_n, tp, n, S, R, length, cg.best.x, failed);
+        candidate_out(w, log_n, S, R, length, cg.best, failed);
         cg.open = false;
     }
     if (!middle) {
@@ -168,8 +171,8 @@ QM_DEV void crest_vertex(PathScratch& w, const Profile& P, CrestGroup& cg, Tops&
         profile_at_walk(P, x, k);
         k.x = x;
         k.z = z;
-        tops_before(w, tp, n, S, R, length, x, failed);
-        candidate_out(w, S, R, length, k, failed);
+        tops_before(w, log_n, tp, n, S, R, length, x, failed);
+        candidate_out(w, log_n, S, R, length, k, failed);
         return;
     }
     if (!cg.open || z > cg.best.z) {
@@ -566,7 +569,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
             lane_states()[s].best_delta = 0.0f;
         }
     }
-    w.log.n = 0;
+    int log_n = 0;
     Tops tp;
     tp.resolved = tp.emitted = 0;
     tp.resolve_x = top_x_from(w, tp.resolved, n);
@@ -589,7 +592,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     P.roof_near = P.head;
     P.roof_x1 = P.roof_top1 = 0.0f;
     foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
-    crest_vertex(w, P, cg, tp, n, S, R, H, 0.0f, ground0, failed);
+    crest_vertex(w, log_n, P, cg, tp, n, S, R, H, 0.0f, ground0, failed);
 
     // The walk's lattice: the coarse one, or with FINE each stretch's in turn.
     Lattice WL = L;
@@ -636,8 +639,21 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
                 if (failed & QM_FAILED_NO_TERRAIN) break;
             }
         } else {
-            if (t_row <= t_column) failed |= row_vertex<FINE>(T, WL, next_row, t, v);
-            else failed |= column_vertex<FINE>(T, WL, next_column, south, t, v);
+            if (FINE) {
+                if (t_row <= t_column) failed |= row_vertex<FINE>(T, WL, next_row, t, v);
+                else failed |= column_vertex<FINE>(T, WL, next_column, south, t, v);
+            } else {
+                // row_vertex or column_vertex with the line's two nodes chosen by selects, so the
+                // lanes of a wave that meet rows and columns in different order read in one pass.
+                const bool on_row = t_row <= t_column;
+                const float c = WL.ca + t * WL.columns;
+                const float west = floorf(c);
+                const float r = WL.ra + t * WL.rise;
+                const int r0 = on_row ? next_row : south, r1 = on_row ? next_row : south + 1;
+                const int c0 = on_row ? (int)west : next_column, c1 = on_row ? (int)west + 1 : next_column;
+                const float f = on_row ? c - west : clampf(r - (float)south, 0.0f, 1.0f);
+                failed |= edge_sample(T, r0, c0, r1, c1, f, v);
+            }
             if (t_row <= t_column) {
                 if (WL.rise > 0.0f) {
                     south = next_row;
@@ -670,7 +686,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         }
         if (x > covered_to) profile_emit(w, tp, n, P, x, z, v.g);
         phase_mark(PHASE_PROFILE);
-        crest_vertex(w, P, cg, tp, n, S, R, H, x, z, failed);
+        crest_vertex(w, log_n, P, cg, tp, n, S, R, H, x, z, failed);
         phase_mark(PHASE_CREST);
     }
     phase_mark(PHASE_CREST);
@@ -679,8 +695,8 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         return;
     }
     if (cg.open) {
-        tops_before(w, tp, n, S, R, H, cg.best.x, failed);
-        candidate_out(w, S, R, H, cg.best, failed);
+        tops_before(w, log_n, tp, n, S, R, H, cg.best.x, failed);
+        candidate_out(w, log_n, S, R, H, cg.best, failed);
     }
     // Tops at the last vertex's x take it; none can lie beyond it.
     while (tp.resolved < n) {
@@ -698,7 +714,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         }
         ++tp.resolved;
     }
-    tops_before(w, tp, n, S, R, H, inf, failed);
+    tops_before(w, log_n, tp, n, S, R, H, inf, failed);
 
     // The direct path (diffraction.rs:46-63) from the whole profile's prefix.
     const Vertex& end = P.head;

@@ -901,15 +901,16 @@ struct Hull {
     CandRef e[HULL_CAP];
 };
 
-// The candidate log of one path: the data of every candidate a state has kept.
+// The candidate log of one path: the data of every candidate a state has kept. Its count is the
+// walk's own register (`log_n`), not a field, so that a kept candidate waits on no memory read.
 struct CandLog {
     CandData entry[LOG_CAP];
-    int n;
 };
 
 // Drops the entries no state refers to any more and renumbers the rest in order (the log is full;
 // at most 2*HULL_CAP + 4 entries are live, so this frees at least 12).
-__device__ __noinline__ void log_compact(CandLog& log, StateStream* states, Hull* hulls) {
+// Returns the new count. Called where a candidate enters (at most one new entry follows).
+__device__ __noinline__ int log_compact(CandLog& log, StateStream* states, Hull* hulls) {
     uint64_t live = 0;
     for (int s = 0; s < 2; ++s) {
         const StateStream& st = states[s];
@@ -920,7 +921,6 @@ __device__ __noinline__ void log_compact(CandLog& log, StateStream* states, Hull
     int next = 0;
     for (int i = 0; i < LOG_CAP; ++i)
         if ((live >> i) & 1ull) log.entry[next++] = log.entry[i];
-    log.n = next;
     for (int s = 0; s < 2; ++s) {
         StateStream& st = states[s];
         for (int i = 0; i < st.hull_n; ++i)
@@ -928,13 +928,14 @@ __device__ __noinline__ void log_compact(CandLog& log, StateStream* states, Hull
         if (st.has_pending) st.pending.id = __popcll(live & ((1ull << st.pending.id) - 1ull));
         if (st.have_best && !st.blocking) st.best.id = __popcll(live & ((1ull << st.best.id) - 1ull));
     }
+    return next;
 }
 
-// The candidate as a state keeps it, logging its data on first use (shared by both states).
-QM_DEV CandRef keep(CandLog& log, StateStream* states, Hull* hulls, Cand& c) {
+// The candidate as a state keeps it, logging its data on first use (shared by both states; the
+// caller leaves a free entry).
+QM_DEV CandRef keep(CandLog& log, int& log_n, Cand& c) {
     if (c.id < 0) {
-        if (log.n >= LOG_CAP) log_compact(log, states, hulls);
-        c.id = log.n++;
+        c.id = log_n++;
         CandData& d = log.entry[c.id];
         d.z_at = c.z_at;
         d.g_at = c.g_at;
@@ -962,16 +963,16 @@ QM_DEV void hull_push(StateStream& st, Hull& hull, P2f S, const CandRef& c, uint
 }
 
 // One candidate (in order of distance) into state `s`'s stream.
-QM_DEV void state_candidate(StateStream* states, Hull* hulls, int s, CandLog& log, P2f S, P2f R, Cand& c,
+QM_DEV void state_candidate(StateStream* states, Hull* hulls, int s, CandLog& log, int& log_n, P2f S, P2f R, Cand& c,
                             uint32_t& failed) {
     StateStream& st = states[s];
     const P2f p{c.x, c.z};
     if (p.z > line_altitude(st.line, p.x)) {
         if (st.has_pending && st.pending.x == p.x) {
-            if (p.z > st.pending.z) st.pending = keep(log, states, hulls, c);
+            if (p.z > st.pending.z) st.pending = keep(log, log_n, c);
         } else {
             if (st.has_pending) hull_push(st, hulls[s], S, st.pending, failed);
-            st.pending = keep(log, states, hulls, c);
+            st.pending = keep(log, log_n, c);
             st.has_pending = true;
         }
         st.blocking = true;
@@ -981,7 +982,7 @@ QM_DEV void state_candidate(StateStream* states, Hull* hulls, int s, CandLog& lo
     const float delta = single_path_difference_f(st.line, st.radius, st.arc_sr, S, p, R);
     if (!st.have_best || delta > st.best_delta) {
         st.best_delta = delta;
-        st.best = keep(log, states, hulls, c);
+        st.best = keep(log, log_n, c);
         st.have_best = true;
     }
 }
