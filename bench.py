@@ -174,8 +174,13 @@ def main():
         samples, (spr, _) = surrogate.generate_samples(gpu, sc, args.band_sources, seed=2, stride=16)
         rows = samples.rows
         d_out = hip.DeviceBuffer(max(rows, 1) * surrogate.OUTPUTS * 4)
-        net.infer_device(samples.inputs.ptr, rows, d_out.ptr)
+        chunk = 16384
+        rows_net = surrogate.Mlp(gpu, np.load(params_file), max_batch=chunk)
+        for at in range(0, rows, chunk):
+            rows_net.infer_device(samples.inputs.ptr + at * surrogate.INPUTS * 4, min(chunk, rows - at),
+                                  d_out.ptr + at * surrogate.OUTPUTS * 4)
         hip.synchronize()
+        rows_net.close()
         band_err = np.abs(d_out.to_array(np.float32, (rows, surrogate.OUTPUTS)).astype(np.float64) -
                           samples.targets.to_array(np.float32, (rows, surrogate.OUTPUTS)).astype(np.float64))
         del samples, d_out
