@@ -883,6 +883,7 @@ struct StateStream {
     float best_delta;
     CandRef pending;
     CandRef best;
+    CandRef top[2];  // the hull's last two entries (top[1] the last), mirrored from its buffer
 };
 
 // Work-items of a group of every kernel that inlines the path walk: 1-D groups of at most this
@@ -910,7 +911,7 @@ struct CandLog {
 // Drops the entries no state refers to any more and renumbers the rest in order (the log is full;
 // at most 2*HULL_CAP + 4 entries are live, so this frees at least 12).
 // Returns the new count. Called where a candidate enters (at most one new entry follows).
-__device__ __noinline__ int log_compact(CandLog& log, StateStream* states, Hull* hulls) {
+QM_DEV int log_compact(CandLog& log, StateStream* states, Hull* hulls) {
     uint64_t live = 0;
     for (int s = 0; s < 2; ++s) {
         const StateStream& st = states[s];
@@ -925,6 +926,8 @@ __device__ __noinline__ int log_compact(CandLog& log, StateStream* states, Hull*
         StateStream& st = states[s];
         for (int i = 0; i < st.hull_n; ++i)
             hulls[s].e[i].id = __popcll(live & ((1ull << hulls[s].e[i].id) - 1ull));
+        if (st.hull_n >= 1) st.top[1] = hulls[s].e[st.hull_n - 1];
+        if (st.hull_n >= 2) st.top[0] = hulls[s].e[st.hull_n - 2];
         if (st.has_pending) st.pending.id = __popcll(live & ((1ull << st.pending.id) - 1ull));
         if (st.have_best && !st.blocking) st.best.id = __popcll(live & ((1ull << st.best.id) - 1ull));
     }
@@ -948,18 +951,25 @@ QM_DEV CandRef keep(CandLog& log, int& log_n, Cand& c) {
 
 QM_DEV void hull_push(StateStream& st, Hull& hull, P2f S, const CandRef& c, uint32_t& failed) {
     while (st.hull_n >= 1) {
-        const P2f a = st.hull_n >= 2 ? P2f{hull.e[st.hull_n - 2].x, hull.e[st.hull_n - 2].z} : S;
-        const P2f b{hull.e[st.hull_n - 1].x, hull.e[st.hull_n - 1].z};
+        const P2f a = st.hull_n >= 2 ? P2f{st.top[0].x, st.top[0].z} : S;
+        const P2f b{st.top[1].x, st.top[1].z};
         const P2f p{c.x, c.z};
         const float ray_at_b = a.z + fdiv((p.z - a.z) * (b.x - a.x), p.x - a.x) + sag_f(st.radius, a, p, b.x - a.x);
-        if (b.z <= ray_at_b) --st.hull_n;
-        else break;
+        if (b.z <= ray_at_b) {
+            --st.hull_n;
+            st.top[1] = st.top[0];
+            if (st.hull_n >= 2) st.top[0] = hull.e[st.hull_n - 2];
+        } else {
+            break;
+        }
     }
     if (st.hull_n >= HULL_CAP) {
         failed |= QM_FAILED_CAPACITY | QM_DETAIL_HULL;
         return;
     }
     hull.e[st.hull_n++] = c;
+    st.top[0] = st.top[1];
+    st.top[1] = c;
 }
 
 // One candidate (in order of distance) into state `s`'s stream.
