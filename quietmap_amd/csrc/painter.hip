@@ -108,7 +108,7 @@ __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(Q
     }
 }
 
-template <bool FINE>
+template <bool FINE, bool FOREST>
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void paint_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, SourceGrid sg, GridArgs g) {
     const Scene& sc = *scene;
     const BandTable& B = *bands;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_
                 const double dx = sx - rx, dy = sy - ry;
                 if (!valid || dx * dx + dy * dy > cut2) continue;
                 PathResult r;
-                evaluate_path<FINE>(sc, B, rcv, sg.sources[s], w, r);
+                evaluate_path<FINE, FOREST>(sc, B, rcv, sg.sources[s], w, r);
                 ++paths;
                 if (r.failed) {
                     ++fails;
@@ -326,18 +326,23 @@ int launch_paint(qm_gpu* gpu, const qm_square* square, const qm_receiver* d_rece
     int per_cu = 0;
     // The general instantiation: fine terrain or the geographic frame's row table.
     const bool fine = square->scene.terrain.fine_detail != nullptr || square->scene.terrain.row_y_m != nullptr;
-    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, fine ? reinterpret_cast<const void*>(paint_kernel<true>) : reinterpret_cast<const void*>(paint_kernel<false>),
-        GROUP, 0));
+    // Coarse squares without forest cover take the instantiation without the foliage term.
+    const void* kernel = fine ? reinterpret_cast<const void*>(paint_kernel<true, true>)
+                         : square->has_forest ? reinterpret_cast<const void*>(paint_kernel<false, true>)
+                                              : reinterpret_cast<const void*>(paint_kernel<false, false>);
+    QM_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, GROUP, 0));
     unsigned groups = (unsigned)std::max(1, cus * std::max(per_cu, 1));
     groups = std::min<unsigned>(groups, g.n_tiles);
     groups = std::max<unsigned>(groups, 1);
     if (fine)
-        hipLaunchKernelGGL(paint_kernel<true>, dim3(groups), dim3(GROUP), 0, stream, square->d_scene, square->d_bands,
-                           square->grid, g);
+        hipLaunchKernelGGL((paint_kernel<true, true>), dim3(groups), dim3(GROUP), 0, stream, square->d_scene,
+                           square->d_bands, square->grid, g);
+    else if (square->has_forest)
+        hipLaunchKernelGGL((paint_kernel<false, true>), dim3(groups), dim3(GROUP), 0, stream, square->d_scene,
+                           square->d_bands, square->grid, g);
     else
-        hipLaunchKernelGGL(paint_kernel<false>, dim3(groups), dim3(GROUP), 0, stream, square->d_scene, square->d_bands,
-                           square->grid, g);
+        hipLaunchKernelGGL((paint_kernel<false, false>), dim3(groups), dim3(GROUP), 0, stream, square->d_scene,
+                           square->d_bands, square->grid, g);
     QM_TRY(hipGetLastError());
     return 0;
 }
@@ -416,6 +421,8 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
     }
     if (upload(sq, canopy, nodes, &d_canopy, err, errlen)) return cleanup(1);
     sq->scene.terrain = *terrain;
+    sq->has_forest = false;
+    for (size_t k = 0; k < nodes && !sq->has_forest; ++k) sq->has_forest = (terrain->nodes[k] >> 24) != 0;
     sq->scene.terrain.nodes = d_nodes;
     sq->scene.terrain.canopy_m = d_canopy;
     sq->scene.terrain.fine_detail = nullptr;

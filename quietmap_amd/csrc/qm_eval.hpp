@@ -421,7 +421,9 @@ struct RayOptions {
 
 // FINE: the scene may hold fine terrain (qm_terrain.fine_detail): the walk's stretches near the
 // ends go over the fine lattice (popup/src/scene.rs:242-309); without it the coarse walk alone.
-template <bool FINE = false>
+// FOREST: the terrain may have forest cover; a square with none takes the instantiation without
+// the foliage term's per-vertex state (its depths stay 0, as they would over open ground).
+template <bool FINE = false, bool FOREST = true>
 QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver& rcv, const qm_source& src,
                           PathScratch& w, PathResult& out, const RayOptions& opt = RayOptions{}) {
     phase_mark(PHASE_OUTSIDE);
@@ -591,7 +593,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
     P.head = Vertex{0.0f, ground0, s0.g, 0.0, 0.0, 0.0};
     P.roof_near = P.head;
     P.roof_x1 = P.roof_top1 = 0.0f;
-    foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
+    if (FOREST) foliage_sample(fol, w, 0.0f, s0.z, s0.forest, s0.canopy, true);
     crest_vertex(w, log_n, P, cg, tp, n, S, R, H, 0.0f, ground0, failed);
 
     // The walk's lattice: the coarse one, or with FINE each stretch's in turn.
@@ -669,7 +671,7 @@ QM_DEV void evaluate_path(const Scene& sc, const BandTable& B, const qm_receiver
         phase_mark(PHASE_VERTEX);
         const float x = t * H;
         const float z = x < platform ? fminf(v.z, source_ground) : v.z;
-        foliage_sample(fol, w, x, v.z, v.forest, v.canopy, false);
+        if (FOREST) foliage_sample(fol, w, x, v.z, v.forest, v.canopy, false);
         // Roofs starting before this vertex, then the vertex unless a roof covers it.
         while (next_roof < x) {
             const Roof f = w.roofs[r];

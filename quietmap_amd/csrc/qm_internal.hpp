@@ -30,6 +30,7 @@ struct qm_square {
     const qm::BandTable* d_bands;    // the GPU's band table in device memory
     SourceGrid grid;  // device pointers
     size_t n_sources;
+    bool has_forest;  // a terrain node with forest cover (else the path kernels skip the foliage term)
     qm_source* d_sources_by_input;  // input order (for pairs)
     std::vector<qm_line_piece> lines;  // line candidates, numbered after the point sources
     std::vector<void*> allocations;

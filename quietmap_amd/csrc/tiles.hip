@@ -49,6 +49,7 @@ struct TilesArgs {
 
 }  // namespace
 
+template <bool FOREST>
 __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void tiles_kernel(
     const Scene* __restrict__ scene, const BandTable* __restrict__ bands, TilesArgs g) {
     const Scene& sc = *scene;
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void
         for (uint32_t s = it.s0; s < it.s1; ++s) {
             if (!valid) continue;
             PathResult r;
-            evaluate_path(sc, B, rcv, g.sources[s], w, r);
+            evaluate_path<false, FOREST>(sc, B, rcv, g.sources[s], w, r);
             if (r.failed) {
                 ++fails;
                 continue;
@@ -208,7 +209,10 @@ int build(qm_tile_plan* p, const qm_square* square, const qm_tile* tiles, size_t
     hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(256), 0, p->stream, d_paths, n_receivers);
     hipLaunchKernelGGL(clear_kernel, dim3(1024), dim3(256), 0, p->stream, d_failures, n_receivers);
     hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, p->stream, p->d_counters, (size_t)XCDS);
-    hipLaunchKernelGGL(tiles_kernel, dim3(groups), dim3(GROUP), 0, p->stream, square->d_scene, square->d_bands, g);
+    if (square->has_forest)
+        hipLaunchKernelGGL(tiles_kernel<true>, dim3(groups), dim3(GROUP), 0, p->stream, square->d_scene, square->d_bands, g);
+    else
+        hipLaunchKernelGGL(tiles_kernel<false>, dim3(groups), dim3(GROUP), 0, p->stream, square->d_scene, square->d_bands, g);
     const hipError_t launched = hipGetLastError();
     const hipError_t ended = hipStreamEndCapture(p->stream, &p->graph);
     T_TRY(launched);

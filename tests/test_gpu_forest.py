@@ -78,3 +78,29 @@ def test_forest_pairs_within_0_05_db(gpu):
     assert err.max() <= TOLERANCE_DB
     assert np.mean(moved > 0.5) > 0.1, "the woods moved too few pairs to test foliage"
     square.close()
+
+
+def test_forest_grid_within_0_05_db(gpu):
+    """The grid painter over the wooded scene (the painter's instantiation with the foliage term;
+    a square without forest cover takes the one without) against the oracle's painted points."""
+    print("case: forest grid", flush=True)
+    sc = scenes.road_scene(receivers=96, lines=60, buildings=120)
+    scene, _ = wooded(sc)
+    nx, ny = sc.grid_shape
+    square = gpu.upload(scene, sc.sources)
+    got = gpu.paint_grid(square, sc.points, nx, ny, sc.cutoff_m)
+    bare_square = gpu.upload(sc.scene, sc.sources)
+    bare = gpu.paint_grid(bare_square, sc.points, nx, ny, sc.cutoff_m)
+    sample = np.random.default_rng(29).choice(sc.points.size, 200, replace=False)
+    want = oracle.paint_points(scene, sc.points[sample], sc.sources, sc.cutoff_m)
+    from quietmap_amd.painter import lden_db
+
+    assert np.array_equal(got["paths"][sample], want["paths"].astype(np.uint32))
+    err = np.abs(lden_db(got["period_energy"][sample]) - lden_db(want["period_energy"]))
+    moved = np.abs(lden_db(got["period_energy"]) - lden_db(bare["period_energy"]))
+    print(f"forest grid: n={err.size} max={np.nanmax(err):.5f} dB; woods move {np.mean(moved > 0.1):.2f} of the "
+          f"receivers by > 0.1 dB", flush=True)
+    assert np.nanmax(err) <= TOLERANCE_DB
+    assert np.mean(moved > 0.1) > 0.1
+    square.close()
+    bare_square.close()
