@@ -499,6 +499,13 @@ int qm_square_upload(qm_gpu* gpu, const qm_terrain* terrain, const qm_obstacles*
         sq->scene.cell_blocks = d_words;
         sq->scene.blocks_x = bx;
         sq->scene.blocks_y = by;
+        std::vector<uint32_t> run_vertex(obstacles->runs_count);
+        for (uint32_t r = 0; r < obstacles->runs_count; ++r)
+            run_vertex[r] = obstacles->outlines[obstacles->runs[r].outline].first_vertex + obstacles->runs[r].first_edge;
+        uint32_t* d_run_vertex = nullptr;
+        if (!run_vertex.empty() && upload(sq, run_vertex.data(), run_vertex.size(), &d_run_vertex, err, errlen))
+            return cleanup(1);
+        sq->scene.run_vertex = d_run_vertex;
     }
     uint8_t* d_weather;
     if (upload(sq, weather->nodes, (size_t)weather->rows * weather->columns * QM_WEATHER_NODE_BYTES, &d_weather, err,

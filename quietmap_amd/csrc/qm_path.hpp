@@ -41,6 +41,9 @@ struct Scene {
     // so the cell walk tests an empty cell from the word it holds instead of reading the index.
     const uint64_t* cell_blocks;
     uint32_t blocks_x, blocks_y;
+    // Built at upload: each run's first vertex (its outline's first vertex plus the run's first
+    // edge), so a cell's edges load right after its runs, the outline record only at a hit.
+    const uint32_t* run_vertex;
 };
 
 struct GroundSample {
@@ -512,8 +515,8 @@ QM_DEV uint32_t find_crossings(const Scene& sc, double fx, double fy, double tx,
             const int32_t low1 = (int32_t)floor(fmin(p1, q1) - pad), high1 = (int32_t)ceil(fmax(p1, q1) + pad);
             for (uint32_t ri = r0; ri < r1; ++ri) {
                 const qm_run run = O.runs[ri];
-                const qm_outline& record = O.outlines[run.outline];
-                const size_t first = (size_t)record.first_vertex + run.first_edge;
+                const size_t first = sc.run_vertex ? (size_t)sc.run_vertex[ri]
+                                                   : (size_t)O.outlines[run.outline].first_vertex + run.first_edge;
                 int32_t ax = O.vertices[2 * first], ay = O.vertices[2 * first + 1];
                 for (size_t next = first + 1; next <= first + run.edge_count; ++next) {
                     const int32_t bx = O.vertices[2 * next], by = O.vertices[2 * next + 1];
@@ -530,6 +533,7 @@ QM_DEV uint32_t find_crossings(const Scene& sc, double fx, double fy, double tx,
                                 if (n >= CROSS_CAP) {
                                     failed |= QM_FAILED_CAPACITY | QM_DETAIL_CROSSINGS;
                                 } else {
+                                    const qm_outline& record = O.outlines[run.outline];
                                     const uint32_t edge = (uint32_t)(next - 1);
                                     int k = n;
                                     while (k > 0 && !crossing_before(out[k - 1], t, record.footprint_id, run.outline, edge)) {
