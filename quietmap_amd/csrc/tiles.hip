@@ -50,7 +50,7 @@ struct TilesArgs {
 }  // namespace
 
 template <bool FOREST>
-__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(2))) void tiles_kernel(
+__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(3))) void tiles_kernel(
     const Scene* __restrict__ scene, const BandTable* __restrict__ bands, TilesArgs g) {
     const Scene& sc = *scene;
     const BandTable& B = *bands;
