@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void receivers_kernel(Scene sc, const qm_point
 // Gpu::evaluate_pairs' ray transfer (one ray per point pair): sums and point bands.
 // FINE: the square holds fine terrain or the geographic frame's row table (the general
 // instantiation, so the coarse kernel keeps its registers).
-template <bool FINE>
+template <bool FINE, bool FOREST>
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(Q
     PathResult r;
     const uint32_t p = pair_point[i];
     const qm_source src = sources[pair_source[i]];
-    evaluate_path<FINE>(sc, B, receivers[p], src, w, r);
+    evaluate_path<FINE, FOREST>(sc, B, receivers[p], src, w, r);
     failed[i] = r.failed;
     float* out = sums + (size_t)i * QM_PAIR_SUMS;
     for (int q = 0; q < PERIODS; ++q) {
@@ -631,18 +631,16 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                           errlen))
         return 1;
     const unsigned blocks = (unsigned)((n_pairs + 255) / 256);
-    if (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m)
-        hipLaunchKernelGGL(pairs_kernel<true>, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
-                           (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
-                           (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
-                           (float*)(pair_bands ? d_pair_bands.p : nullptr));
-    else
-        hipLaunchKernelGGL(pairs_kernel<false>, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
-                           (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
-                           (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                           (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
-                           (float*)(pair_bands ? d_pair_bands.p : nullptr));
+    // The general instantiation for fine terrain or a row table; coarse squares without forest
+    // cover without the foliage term.
+    auto kernel = (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m) ? pairs_kernel<true, true>
+                  : square->has_forest                                                  ? pairs_kernel<false, true>
+                                                                                        : pairs_kernel<false, false>;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
+                       (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
+                       (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
+                       (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
+                       (float*)(pair_bands ? d_pair_bands.p : nullptr));
     QM_TRY(hipGetLastError());
     QM_TRY(hipMemcpyAsync(sums, d_sums.p, sizeof(float) * n_pairs * QM_PAIR_SUMS, hipMemcpyDeviceToHost, stream));
     QM_TRY(hipMemcpyAsync(failed, d_failed.p, sizeof(uint32_t) * n_pairs, hipMemcpyDeviceToHost, stream));
