@@ -81,7 +81,9 @@ def test_geo_pairs_and_grid(gpu, geo):
     sample = rng.choice(geo.points.size, 256, replace=False)
     ref = oracle.paint_points(geo.scene, geo.points[sample], geo.sources, 1e9)
     assert np.array_equal(g["paths"][sample].astype(np.int64), ref["paths"].astype(np.int64))
-    e = np.abs(lden_db(g["period_energy"][sample]) - lden_db(ref["period_energy"]))
+    got_l, want_l = lden_db(g["period_energy"][sample]), lden_db(ref["period_energy"])
+    assert np.array_equal(np.isfinite(got_l), np.isfinite(want_l))
+    e = np.abs(got_l - want_l)
     e = e[np.isfinite(e)]
     print(f"geo grid Lden: n={e.size} max {e.max():.5f} dB", flush=True)
     assert e.max() <= 0.05

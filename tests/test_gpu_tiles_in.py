@@ -68,7 +68,9 @@ def test_square_from_tiles(gpu):
     pick = rng.choice(points.size, 128, replace=False)
     ref = oracle.paint_points(scene, points[pick], src, 800.0)
     assert np.array_equal(r["paths"][pick].astype(np.int64), ref["paths"].astype(np.int64))
-    e = np.abs(lden_db(r["period_energy"][pick]) - lden_db(ref["period_energy"]))
+    got_l, want_l = lden_db(r["period_energy"][pick]), lden_db(ref["period_energy"])
+    assert np.array_equal(np.isfinite(got_l), np.isfinite(want_l))
+    e = np.abs(got_l - want_l)
     e = e[np.isfinite(e)]
     print(f"grid from tiles: Lden max {e.max():.5f} dB", flush=True)
     assert e.max() <= 0.05
