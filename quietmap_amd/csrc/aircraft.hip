@@ -10,10 +10,12 @@
 //
 // A work-item per receiver; a 256-item group is a 16x16 tile of receivers (four 8x8 waves) pulling
 // tiles from per-XCD queues. Tracks and their segments are uniform across the group, read with
-// scalar loads. SEL energies run in FP32 (per-track sums, FP64 across tracks). An event's LAmax is
-// the maximum over its segments: the FP32 pass keeps the three loudest segments, and those within
-// 0.02 dB of the loudest are re-read in FP64 with the oracle's operation order (no contraction),
-// so thresholds and histogram bins are decided on the same FP64 value the CPU path computes.
+// scalar loads. SEL energies run in FP32 (per-track sums, FP64 across tracks) on the hardware
+// reciprocal, log2 and exp2. An event's LAmax is the maximum over its segments: the FP32 pass
+// keeps the three loudest segments; where that maximum lies within 0.02 dB of a 0.5 dB edge (the
+// histogram's, 65 and 70 dB among them), the segments within 0.02 dB of the loudest are re-read in
+// FP64 with the oracle's operation order (no contraction), so thresholds and histogram bins are
+// decided on the same side of every edge as the FP64 value the CPU path computes.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,6 +34,9 @@ namespace {
 using namespace qm_doc29;
 
 constexpr float EPS_DB = 0.02f;
+// N65 and N70 are decided at edges of the 0.5 dB histogram grid, so one edge test covers both.
+static_assert((65.0 - QM_HIST_FLOOR_DB) / QM_HIST_STEP_DB == 50.0 && (70.0 - QM_HIST_FLOOR_DB) / QM_HIST_STEP_DB == 60.0,
+              "65 and 70 dB lie on the histogram's edges");
 
 // One power row as the device reads it (npd.rs:256-330), FP32 and FP64 copies.
 struct DevRow {
@@ -91,14 +96,14 @@ struct AircraftArgs {
 // FP32 reads.
 __device__ __forceinline__ void position_f(const Tables& T, float slant, int& interval, float& fraction, float& s) {
     s = fmaxf(slant, (float)NPD_NEAREST_SLANT_M);
-    const float log_d = log10f(s / (float)METRES_PER_FOOT);
+    const float log_d = log10_f(s * (float)(1.0 / METRES_PER_FOOT));
     interval = 0;
     for (int k = ND - 2; k >= 1; --k)
         if (log_d > T.log_npd[k]) {
             interval = k;
             break;
         }
-    fraction = (log_d - T.log_npd[interval]) / (T.log_npd[interval + 1] - T.log_npd[interval]);
+    fraction = div_f(log_d - T.log_npd[interval], T.log_npd[interval + 1] - T.log_npd[interval]);
 }
 
 __device__ __forceinline__ float linear_f(const float* c, int i, float f) { return c[i] + f * (c[i + 1] - c[i]); }
@@ -119,7 +124,7 @@ __device__ double segment_lmax_d(const AircraftArgs& g, const DevTrack& tr, cons
         lateral = hypot(s[0], s[1]);
         height = s[2];
     } else {
-        along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / ls;
+        along = div_f(-(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]), ls);
         const double cl = fmin(fmax(along, 0.0), 1.0);
         for (int k = 0; k < 3; ++k) {
             on_line[k] = s[k] + along * d[k];
@@ -287,7 +292,7 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     const float reach = (float)g.reach_m;
     {
         const float ll = d[0] * d[0] + d[1] * d[1];
-        const float t = ll > 0.0f ? fminf(fmaxf(-(s[0] * d[0] + s[1] * d[1]) / ll, 0.0f), 1.0f) : 0.0f;
+        const float t = ll > 0.0f ? fminf(fmaxf(div_f(-(s[0] * d[0] + s[1] * d[1]), ll), 0.0f), 1.0f) : 0.0f;
         const float px = s[0] + t * d[0], py = s[1] + t * d[1];
         const float q = px * px + py * py;
         if (q > (reach + 1.0f) * (reach + 1.0f)) return out;
@@ -325,23 +330,23 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     {
         const float sml = linear_f(lo.sml, interval, fraction);
         float s_lo = ps < (float)NPD_LAST_DISTANCE_M ? linear_f(lo.sel, interval, fraction)
-                                                    : lo.sel[ND - 1] - 20.0f * log10f(ps / (float)NPD_LAST_DISTANCE_M) -
+                                                    : lo.sel[ND - 1] - 20.0f * log10_f(ps * (float)(1.0 / NPD_LAST_DISTANCE_M)) -
                                                           lo.tail_alpha * (ps - (float)NPD_LAST_DISTANCE_M);
-        float d_lo = lo.dipole ? ps : lo.d0 * exp10f(sml * 0.1f);
+        float d_lo = lo.dipole ? ps : lo.d0 * exp10_f(sml * 0.1f);
         if (tr.weight != 0.0f) {
             const DevRow& hi = rows[tr.row_index + 1];
             const float smh = linear_f(hi.sml, interval, fraction);
             const float s_hi = ps < (float)NPD_LAST_DISTANCE_M
                                    ? linear_f(hi.sel, interval, fraction)
-                                   : hi.sel[ND - 1] - 20.0f * log10f(ps / (float)NPD_LAST_DISTANCE_M) -
+                                   : hi.sel[ND - 1] - 20.0f * log10_f(ps * (float)(1.0 / NPD_LAST_DISTANCE_M)) -
                                          hi.tail_alpha * (ps - (float)NPD_LAST_DISTANCE_M);
-            const float d_hi = hi.dipole ? ps : hi.d0 * exp10f(smh * 0.1f);
+            const float d_hi = hi.dipole ? ps : hi.d0 * exp10_f(smh * 0.1f);
             s_lo = s_lo + tr.weight * (s_hi - s_lo);
             d_lo = d_lo + tr.weight * (d_hi - d_lo);
         }
         sel = ps < (float)NPD_LAST_DISTANCE_M
                   ? s_lo
-                  : tr.sel_last - 20.0f * log10f(ps / (float)NPD_LAST_DISTANCE_M) -
+                  : tr.sel_last - 20.0f * log10_f(ps * (float)(1.0 / NPD_LAST_DISTANCE_M)) -
                         tr.sel_alpha * (ps - (float)NPD_LAST_DISTANCE_M);
         scaled = d_lo;
     }
@@ -358,7 +363,7 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
                                                on_seg[1], on_seg[2]);
         screening = fmaxf(fmaxf(terrain, buildings) - lat, 0.0f);
     }
-    out.sel_energy = exp10f((sel + tr.dv - lat - screening) * 0.1f) * inst * finite;
+    out.sel_energy = exp10_f((sel + tr.dv - lat - screening) * 0.1f) * inst * finite;
     // LAmax (segment.rs:357-381).
     const float distance = sqrtf(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1] + on_seg[2] * on_seg[2]);
     float lamax;
@@ -373,14 +378,16 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
                 lamax = lamax + tr.weight * (linear_f(hi.lamax, li, lf) - lamax);
             }
         } else {
-            lamax = tr.lamax_last - 20.0f * log10f(lps / (float)NPD_LAST_DISTANCE_M) -
+            lamax = tr.lamax_last - 20.0f * log10_f(lps * (float)(1.0 / NPD_LAST_DISTANCE_M)) -
                     tr.lamax_alpha * (lps - (float)NPD_LAST_DISTANCE_M);
         }
     }
+    // Within the segment's span the event reads the lateral attenuation at the closest point,
+    // the SEL's own (lambda_height is the height there).
     const float lat2 = (along >= 0.0f && along <= 1.0f)
-                           ? lateral_f(height, lateral)
+                           ? lat
                            : lateral_f(on_seg[2], sqrtf(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1]));
-    out.lmax = lamax + 10.0f * log10f(installation_f(tr.installation, height, slant)) - lat2;
+    out.lmax = lamax + 10.0f * log10_f(installation_f(tr.installation, height, slant)) - lat2;
     return out;
 }
 
@@ -467,7 +474,7 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
                         const double d0 = (double)pa[3] - (double)pa[0], d1 = (double)pa[4] - (double)pa[1],
                                      d2 = (double)pa[5] - (double)pa[2];
                         const double length = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                        const float level = 10.0f * log10f(sf.sel_energy / (float)(length * tr.seconds_per_metre));
+                        const float level = 10.0f * log10_f(div_f(sf.sel_energy, (float)(length * tr.seconds_per_metre)));
                         const float position = (level - (float)QM_PASS_LOWEST_DB) / (float)QM_PASS_CLASS_DB;
                         const int fast_class = min((int)fmaxf(position, 0.0f), QM_PASS_CLASSES - 1);
                         const float edge = rintf(position);
@@ -509,11 +516,18 @@ __global__ __launch_bounds__(256) void aircraft_kernel(AircraftArgs g) {
             if (!any) continue;
             for (int p = 0; p < 3; ++p) energy[p] += (double)tr.flights[p] * (double)e_track;
             if (m1 < (float)QM_HIST_FLOOR_DB - EPS_DB) continue;
-            // The event's LAmax in FP64 over its candidate segments.
+            // The FP32 maximum decides N65, N70 and the bin wherever it lies EPS_DB or more from
+            // every 0.5 dB edge (65 and 70 dB are such edges): the FP64 maximum is then on the same
+            // side of each. Within EPS_DB of one, the event's LAmax is re-read in FP64 over its
+            // candidate segments.
+            const float steps = rintf((m1 - (float)QM_HIST_FLOOR_DB) * (float)(1.0 / QM_HIST_STEP_DB));
+            const bool near_edge = fabsf(m1 - ((float)QM_HIST_FLOOR_DB + (float)QM_HIST_STEP_DB * steps)) < EPS_DB;
+            double lmax = (double)m1;
             const float* v = g.vertices + 3 * (size_t)tr.first_vertex;
-            double lmax = segment_lmax_d(g, tr, g.rows, v + 3 * i1, v + 3 * i1 + 3, rx, ry);
-            if (m2 >= m1 - EPS_DB) lmax = fmax(lmax, segment_lmax_d(g, tr, g.rows, v + 3 * i2, v + 3 * i2 + 3, rx, ry));
-            if (m3 >= m1 - EPS_DB) {
+            if (near_edge) lmax = segment_lmax_d(g, tr, g.rows, v + 3 * i1, v + 3 * i1 + 3, rx, ry);
+            if (near_edge && m2 >= m1 - EPS_DB)
+                lmax = fmax(lmax, segment_lmax_d(g, tr, g.rows, v + 3 * i2, v + 3 * i2 + 3, rx, ry));
+            if (near_edge && m3 >= m1 - EPS_DB) {
                 // Three or more within reach of the maximum: every such segment again.
                 for (uint32_t k = 0; k + 1 < tr.vertex_count; ++k) {
                     const SegmentF sf = segment_f(g, tr, g.rows, v + 3 * k, v + 3 * k + 3, rxf, ryf, rx, ry, nullptr);

@@ -21,6 +21,13 @@ constexpr double NPD_LAST_DISTANCE_M = 25000.0 * METRES_PER_FOOT;
 constexpr double TAIL_ANCHOR_M = 16000.0;
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
+// The fast pass's FP32 arithmetic on the hardware reciprocal, log2 and exp2 (about 1 ulp each, no
+// IEEE division sequences): its error stays orders of magnitude under the 0.02 dB margin within
+// which every decision (pass class, N65 / N70, histogram bin) is re-read in FP64.
+__device__ __forceinline__ float div_f(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float log10_f(float x) { return __builtin_amdgcn_logf(x) * 0.301029995663981195f; }
+__device__ __forceinline__ float exp10_f(float x) { return __builtin_amdgcn_exp2f(x * 3.32192809488736235f); }
+
 // corrections.rs:36-45: (e(a1) - e(a0)) / pi with e(a) = a / (1 + a^2) + atan(a), a1 - a0 =
 // length / scaled. Where both ends lie two or more scaled distances to one side, e is within 0.03
 // of pi / 2 at both and the FP32 difference loses its digits (a segment far behind the receiver
@@ -30,10 +37,11 @@ constexpr double PI_D = 3.141592653589793238462643383279502884;
 // (u0 - u1) sum_k c_k P_2k, P_n = (u0^(n+1) - u1^(n+1)) / (u0 - u1), u <= 1/2.
 __device__ __forceinline__ float finite_fraction_f(float along, float length, float scaled) {
     if (length < 1.0f || scaled < 1.0f) return 1.0f;
-    const float a0 = -along / scaled, a1 = -(along - length) / scaled;
+    const float inv_scaled = __builtin_amdgcn_rcpf(scaled);
+    const float a0 = -along * inv_scaled, a1 = -(along - length) * inv_scaled;
     if (a0 * a1 > 0.0f && fminf(fabsf(a0), fabsf(a1)) >= 2.0f) {
-        const float u0 = 1.0f / fminf(fabsf(a0), fabsf(a1)), u1 = 1.0f / fmaxf(fabsf(a0), fabsf(a1));
-        const float du = (length / scaled) / (a0 * a1);
+        const float u0 = __builtin_amdgcn_rcpf(fminf(fabsf(a0), fabsf(a1))), u1 = __builtin_amdgcn_rcpf(fmaxf(fabsf(a0), fabsf(a1)));
+        const float du = div_f(length * inv_scaled, a0 * a1);
         float p = 1.0f, power = 1.0f, sum = 0.0f, sign = 1.0f;
 #pragma unroll
         for (int n = 1; n <= 28; ++n) {
@@ -47,13 +55,13 @@ __device__ __forceinline__ float finite_fraction_f(float along, float length, fl
         }
         return fmaxf(du * sum * (float)(1.0 / PI_D), 1e-15f);
     }
-    const float e0 = a0 / (1.0f + a0 * a0) + atanf(a0);
-    const float e1 = a1 / (1.0f + a1 * a1) + atanf(a1);
+    const float e0 = div_f(a0, 1.0f + a0 * a0) + atanf(a0);
+    const float e1 = div_f(a1, 1.0f + a1 * a1) + atanf(a1);
     return fmaxf((e1 - e0) * (float)(1.0 / PI_D), 1e-15f);
 }
 
 __device__ __forceinline__ float lateral_f(float height, float lateral) {
-    const float beta = atanf(height / fmaxf(lateral, 0.01f)) * (float)(180.0 / PI_D);
+    const float beta = atanf(div_f(height, fmaxf(lateral, 0.01f))) * (float)(180.0 / PI_D);
     if (beta > 50.0f) return 0.0f;
     const float gamma = lateral <= 914.0f ? 1.089f * (1.0f - __expf(-0.00274f * lateral)) : 1.0f;
     const float lb = beta < 0.0f ? 10.857f : 1.137f - 0.0229f * beta + 9.72f * __expf(-0.142f * beta);
@@ -66,12 +74,12 @@ __device__ __forceinline__ float installation_f(uint32_t inst, float height, flo
     const float b = inst == QM_INSTALL_WING ? 0.062f : 0.329f;
     const float c = inst == QM_INSTALL_WING ? 0.8786f : 1.0f;
     height = fmaxf(height, 0.0f);
-    const float sin2 = height * height / fmaxf(slant * slant, 1e-12f);
+    const float sin2 = div_f(height * height, fmaxf(slant * slant, 1e-12f));
     const float cos2 = 1.0f - sin2;
     const float num = a * cos2 + sin2;
     const float d = cos2 - sin2;
     const float den = c * 4.0f * sin2 * cos2 + d * d;
-    return (den > 0.0f && num > 0.0f) ? __powf(num, b) / den : 1.0f;
+    return (den > 0.0f && num > 0.0f) ? div_f(__powf(num, b), den) : 1.0f;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -150,7 +158,7 @@ inline __device__ double horizon_loss_d(const float* at, int sectors, const doub
 // 2 (u x v)^2 / ((|u||v| + u.v)(|u| + |v| + |u + v|)), u = RE, v = ES, which keeps its digits
 // where the direct sum of three kilometre lengths loses the metre-scale difference.
 __device__ __forceinline__ float path_difference_loss_f(float delta) {
-    const float raw = 10.0f * log10f(3.0f + (float)QM_PATH_DIFFERENCE_SLOPE_PER_M * fmaxf(delta, 0.0f));
+    const float raw = 10.0f * log10_f(3.0f + (float)QM_PATH_DIFFERENCE_SLOPE_PER_M * fmaxf(delta, 0.0f));
     return fminf(fmaxf(raw - (float)QM_GRAZING_LOSS_DB, 0.0f), (float)QM_SCREENING_CAP_DB);
 }
 
@@ -159,7 +167,7 @@ __device__ __forceinline__ float edge_loss_f(float er, float eh, float sr, float
     const float vx = sr - er, vy = sh - eh;
     const float a = sqrtf(er * er + eh * eh), b = sqrtf(vx * vx + vy * vy), c = sqrtf(sr * sr + sh * sh);
     const float cross = er * vy - eh * vx, dot = er * vx + eh * vy;
-    return path_difference_loss_f(2.0f * cross * cross / ((a * b + dot) * (a + b + c)));
+    return path_difference_loss_f(div_f(2.0f * cross * cross, (a * b + dot) * (a + b + c)));
 }
 
 // horizons.rs:84-104 SectorEdges::loss_db in FP32 over the card's floats.
