@@ -124,7 +124,7 @@ __device__ double segment_lmax_d(const AircraftArgs& g, const DevTrack& tr, cons
         lateral = hypot(s[0], s[1]);
         height = s[2];
     } else {
-        along = div_f(-(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]), ls);
+        along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / ls;
         const double cl = fmin(fmax(along, 0.0), 1.0);
         for (int k = 0; k < 3; ++k) {
             on_line[k] = s[k] + along * d[k];
@@ -304,11 +304,11 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     if (hs <= 1e-6f) {
         along = 0.0f;
         for (int k = 0; k < 3; ++k) on_line[k] = on_seg[k] = s[k];
-        lateral = sqrtf(s[0] * s[0] + s[1] * s[1]);
+        lateral = sqrt_f(s[0] * s[0] + s[1] * s[1]);
         height = s[2];
         lambda_height = s[2];
     } else {
-        along = -(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]) / ls;
+        along = div_f(-(s[0] * d[0] + s[1] * d[1] + s[2] * d[2]), ls);
         const float cl = fminf(fmaxf(along, 0.0f), 1.0f);
         for (int k = 0; k < 3; ++k) {
             on_line[k] = s[k] + along * d[k];
@@ -316,11 +316,11 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
         }
         lateral = fabsf(s[0] * d[1] - s[1] * d[0]) * rsqrtf(hs);
         const float ss = on_line[0] * on_line[0] + on_line[1] * on_line[1] + on_line[2] * on_line[2];
-        height = copysignf(sqrtf(fmaxf(ss - lateral * lateral, 0.0f)), on_line[2]);
-        lambda_height = (along >= 0.0f && along <= 1.0f) ? height : on_seg[2] * sqrtf(ls / hs);
+        height = copysignf(sqrt_f(fmaxf(ss - lateral * lateral, 0.0f)), on_line[2]);
+        lambda_height = (along >= 0.0f && along <= 1.0f) ? height : on_seg[2] * sqrt_f(div_f(ls, hs));
     }
-    const float length = fmaxf(sqrtf(ls), 1.0f);
-    const float slant = sqrtf(on_line[0] * on_line[0] + on_line[1] * on_line[1] + on_line[2] * on_line[2]);
+    const float length = fmaxf(sqrt_f(ls), 1.0f);
+    const float slant = sqrt_f(on_line[0] * on_line[0] + on_line[1] * on_line[1] + on_line[2] * on_line[2]);
     // SegmentEmission::read_npd (segment.rs:126-146).
     int interval;
     float fraction, ps;
@@ -365,7 +365,7 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     }
     out.sel_energy = exp10_f((sel + tr.dv - lat - screening) * 0.1f) * inst * finite;
     // LAmax (segment.rs:357-381).
-    const float distance = sqrtf(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1] + on_seg[2] * on_seg[2]);
+    const float distance = sqrt_f(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1] + on_seg[2] * on_seg[2]);
     float lamax;
     {
         int li;
@@ -386,7 +386,7 @@ __device__ __forceinline__ SegmentF segment_f(const AircraftArgs& g, const DevTr
     // the SEL's own (lambda_height is the height there).
     const float lat2 = (along >= 0.0f && along <= 1.0f)
                            ? lat
-                           : lateral_f(on_seg[2], sqrtf(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1]));
+                           : lateral_f(on_seg[2], sqrt_f(on_seg[0] * on_seg[0] + on_seg[1] * on_seg[1]));
     out.lmax = lamax + 10.0f * log10_f(installation_f(tr.installation, height, slant)) - lat2;
     return out;
 }

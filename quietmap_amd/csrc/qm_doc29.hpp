@@ -27,6 +27,7 @@ constexpr double PI_D = 3.141592653589793238462643383279502884;
 __device__ __forceinline__ float div_f(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float log10_f(float x) { return __builtin_amdgcn_logf(x) * 0.301029995663981195f; }
 __device__ __forceinline__ float exp10_f(float x) { return __builtin_amdgcn_exp2f(x * 3.32192809488736235f); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // corrections.rs:36-45: (e(a1) - e(a0)) / pi with e(a) = a / (1 + a^2) + atan(a), a1 - a0 =
 // length / scaled. Where both ends lie two or more scaled distances to one side, e is within 0.03
@@ -165,14 +166,14 @@ __device__ __forceinline__ float path_difference_loss_f(float delta) {
 __device__ __forceinline__ float edge_loss_f(float er, float eh, float sr, float sh) {
     if (!(er < sr && eh * sr > sh * er)) return 0.0f;
     const float vx = sr - er, vy = sh - eh;
-    const float a = sqrtf(er * er + eh * eh), b = sqrtf(vx * vx + vy * vy), c = sqrtf(sr * sr + sh * sh);
+    const float a = sqrt_f(er * er + eh * eh), b = sqrt_f(vx * vx + vy * vy), c = sqrt_f(sr * sr + sh * sh);
     const float cross = er * vy - eh * vx, dot = er * vx + eh * vy;
     return path_difference_loss_f(div_f(2.0f * cross * cross, (a * b + dot) * (a + b + c)));
 }
 
 // horizons.rs:84-104 SectorEdges::loss_db in FP32 over the card's floats.
 inline __device__ float horizon_loss_f(const float* at, int sectors, float x, float y, float z) {
-    const float range = sqrtf(x * x + y * y);
+    const float range = sqrt_f(x * x + y * y);
     if (range <= 1.0f || z >= at[0] * range) return 0.0f;
     float angle = atan2f(y, x);
     if (angle < 0.0f) angle += (float)(2.0 * PI_D);
