@@ -16,8 +16,10 @@
 //   37      0 calm, 1 favourable (the caller's; this function writes 0)
 //   38      G under the source (as the exact path takes it)
 //   39      the roofed fraction of the path
-// The envelope the network is trained and used in: no failure, length >= 200 m, at most 8 wall
-// tops, no forest at the samples or the ends. Outside it the exact path is the answer.
+// The envelope the network is trained and used in: no failure, length >= 500 m, at most 8 wall
+// tops, no forest at the samples or the ends. Outside it the exact path is the answer: the paths
+// within 500 m carry much of a receiver's energy, so the exact path keeps them and the network's
+// per-band error moves the receiver's level less.
 #pragma once
 
 #include "qm_eval.hpp"
@@ -26,7 +28,7 @@ namespace qm {
 
 constexpr int FEATURES = 40;
 constexpr int FEATURE_SAMPLES = 48;
-constexpr float ENVELOPE_MIN_LENGTH_M = 200.0f;
+constexpr float ENVELOPE_MIN_LENGTH_M = 500.0f;
 constexpr int ENVELOPE_TOPS = 8;
 
 struct SurrogatePath {

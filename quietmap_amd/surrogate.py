@@ -100,7 +100,7 @@ class Mlp:
 # ---------------------------------------------------------------------------------------------
 # Samples, training and the surrogate painter.
 
-MIN_LENGTH_M = 200.0  # the envelope's shortest path (qm_features.hpp)
+MIN_LENGTH_M = 500.0  # the envelope's shortest path (qm_features.hpp)
 
 
 def samples_device(gpu, square, d_receivers, d_pair_receiver, d_pair_source, n_pairs, d_inputs, d_targets,

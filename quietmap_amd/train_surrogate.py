@@ -4,7 +4,7 @@
 
 Samples: paths of a tile-type scene (seed 11, a different scene from the benchmark's seed 2) by
 the exact kernel: `--sources` random sources, each with every `--stride`-th receiver row and column
-between 200 m and the 2 km cutoff, kept where the path lies in the network's envelope; two rows a
+between 500 m and the 2 km cutoff, kept where the path lies in the network's envelope; two rows a
 path (calm, favourable), the exact A_ground + A_dif per band as targets. Training: Adam in
 shuffled batches, the output biases starting at the per-band target means. Prints one line per
 100 steps and a JSON summary (rows, steps, losses, seconds).

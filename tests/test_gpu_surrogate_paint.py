@@ -90,7 +90,7 @@ def features_reference(sc, point, altitude, source):
     f[36] = len(tops) / 8
     f[38] = source["ground_percent"] / 100 if source["ground_percent"] != 255 else s0[1]
     f[39] = min(roofed / H, 1.0)
-    envelope = failed == 0 and H >= 200 and len(tops) <= 8 and not forest
+    envelope = failed == 0 and H >= 500 and len(tops) <= 8 and not forest
     return f, envelope
 
 
