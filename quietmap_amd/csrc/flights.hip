@@ -267,8 +267,10 @@ __device__ int pass_class_exact(double energy, double passing) {
 
 // The arrays come as kernel arguments of their own (global pointers), the tables in `g`: pointers
 // read out of an argument struct that is also passed by reference were generic, and every box
-// field a flat load.
-__global__ __launch_bounds__(GROUP) void flight_sums_kernel(const qm_flight_box* __restrict__ boxes,
+// field a flat load. At 4 waves per SIMD (128 VGPRs, some spilled around the FP64 re-read) the
+// launch on 20,000 boxes x 4,096 points takes 21.2 ms against 26.2 ms at the 2 its 203 VGPRs
+// would give (21.8 ms at 3): the box reads' latency, not the spills, bounds it.
+__global__ __launch_bounds__(GROUP) __attribute__((amdgpu_waves_per_eu(4))) void flight_sums_kernel(const qm_flight_box* __restrict__ boxes,
                                                             const qm_flight_point* __restrict__ points,
                                                             const float* __restrict__ horizons,
                                                             const qm_flight_segment* __restrict__ segments,
