@@ -2,7 +2,12 @@
 //
 // Kernels (gfx950, wave64):
 //   receivers_kernel  - paint-gpu kernels' point_receivers: ground + 4 m and the weather there
-//   pairs_kernel      - Gpu::evaluate_pairs' ray transfers: a work-item per pair
+//   pairs_kernel      - Gpu::evaluate_pairs' ray transfers: a work-item per pair, the pairs taken
+//                       in a coherent order (by source, then the receiver's Morton code on an 8 m
+//                       grid, from pair_order_kernel and a device radix sort) so that a wave's
+//                       rays leave one source for a compact block of receivers; each pair's
+//                       results land at its own index
+//   pair_order_kernel - the sort keys of that order
 //   paint_kernel      - the grid painter: a persistent launch of 256-work-item groups, each group
 //                       a 16x16 block of receivers (four 8x8 waves: neighbouring receivers in one
 //                       wave follow near-identical rays), pulling tiles from per-XCD queues and
@@ -10,6 +15,7 @@
 //                       bucket grid around the tile, uniform across the group (scalar loads).
 //   crossings_kernel  - the exact crossing list of segments (parity hook)
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -75,12 +81,13 @@ template <bool FINE, bool FOREST>
 __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(QM_PAINT_WAVES_PER_EU))) void pairs_kernel(const Scene* __restrict__ scene, const BandTable* __restrict__ bands, const qm_receiver* receivers,
                                                     const qm_source* sources, const uint32_t* pair_point,
                                                     const uint32_t* pair_source, const float* pair_weight,
-                                                    uint32_t n, float* sums, uint32_t* failed, float* point_bands,
-                                                    float* pair_bands) {
+                                                    const uint32_t* order, uint32_t n, float* sums, uint32_t* failed,
+                                                    float* point_bands, float* pair_bands) {
     const Scene& sc = *scene;
     const BandTable& B = *bands;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order ? order[k] : k;
     PathScratch w;
     PathResult r;
     const uint32_t p = pair_point[i];
@@ -106,6 +113,30 @@ __global__ __launch_bounds__(QM_PATH_GROUP) __attribute__((amdgpu_waves_per_eu(Q
         for (int q = 0; q < PERIODS; ++q)
             for (int b = 0; b < BANDS; ++b) atomicAdd(dst + q * BANDS + b, weight * r.bands[q][b]);
     }
+}
+
+// The bits of x interleaved with zeros (bit b to bit 2b), for a 16-bit Morton code.
+__device__ __forceinline__ uint32_t spread_bits16(uint32_t x) {
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// Sort keys of the pairs' coherent order: the source in the high word, then the Morton code of the
+// receiver's position on an 8 m grid (modulo 524 km a side, which only a square far larger than any
+// tile would wrap); the values are the pairs' own indices.
+__global__ __launch_bounds__(256) void pair_order_kernel(const qm_receiver* receivers, const uint32_t* pair_point,
+                                                         const uint32_t* pair_source, uint32_t n, uint64_t* keys,
+                                                         uint32_t* indices) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const qm_receiver& r = receivers[pair_point[i]];
+    const uint32_t qx = (uint32_t)(int64_t)floor(r.x_m * 0.125), qy = (uint32_t)(int64_t)floor(r.y_m * 0.125);
+    keys[i] = ((uint64_t)pair_source[i] << 32) | (spread_bits16(qy) << 1) | spread_bits16(qx);
+    indices[i] = i;
 }
 
 template <bool FINE, bool FOREST>
@@ -591,7 +622,7 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                       size_t errlen) {
     if (!gpu || !square || (n_pairs && (!pair_point || !pair_source || !sums || !failed)))
         return fail(err, errlen, "null argument");
-    if (n_pairs > 0xffffffffu || n_points > 0xffffffffu) return fail(err, errlen, "too many pairs or points");
+    if (n_pairs > 0x7fffffffu || n_points > 0xffffffffu) return fail(err, errlen, "too many pairs or points");
     for (size_t i = 0; i < n_points; ++i)
         if (!std::isfinite(points[i].x_m) || !std::isfinite(points[i].y_m))
             return fail(err, errlen, "point %zu: position not finite", i);
@@ -631,6 +662,28 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                           errlen))
         return 1;
     const unsigned blocks = (unsigned)((n_pairs + 255) / 256);
+    // The coherent order: keys, then a radix sort over the source's significant bits and the
+    // 32-bit Morton code (the sort is stable, so pairs of one receiver and source keep their order).
+    DeviceBuffer d_keys, d_keys_sorted, d_index, d_order, d_sort_temp;
+    if (device_alloc(d_keys, sizeof(uint64_t) * n_pairs, err, errlen) ||
+        device_alloc(d_keys_sorted, sizeof(uint64_t) * n_pairs, err, errlen) ||
+        device_alloc(d_index, sizeof(uint32_t) * n_pairs, err, errlen) ||
+        device_alloc(d_order, sizeof(uint32_t) * n_pairs, err, errlen))
+        return 1;
+    hipLaunchKernelGGL(pair_order_kernel, dim3(blocks), dim3(256), 0, stream, (const qm_receiver*)d_receivers.p,
+                       (const uint32_t*)d_pp.p, (const uint32_t*)d_ps.p, (uint32_t)n_pairs, (uint64_t*)d_keys.p,
+                       (uint32_t*)d_index.p);
+    QM_TRY(hipGetLastError());
+    int source_bits = 1;
+    while (source_bits < 32 && (square->n_sources >> source_bits) != 0) ++source_bits;
+    size_t sort_bytes = 0;
+    QM_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t*)d_keys.p, (uint64_t*)d_keys_sorted.p,
+                                              (const uint32_t*)d_index.p, (uint32_t*)d_order.p, (int)n_pairs, 0,
+                                              32 + source_bits, stream));
+    if (device_alloc(d_sort_temp, sort_bytes, err, errlen)) return 1;
+    QM_TRY(hipcub::DeviceRadixSort::SortPairs(d_sort_temp.p, sort_bytes, (const uint64_t*)d_keys.p,
+                                              (uint64_t*)d_keys_sorted.p, (const uint32_t*)d_index.p,
+                                              (uint32_t*)d_order.p, (int)n_pairs, 0, 32 + source_bits, stream));
     // The general instantiation for fine terrain or a row table; coarse squares without forest
     // cover without the foliage term.
     auto kernel = (square->scene.terrain.fine_detail || square->scene.terrain.row_y_m) ? pairs_kernel<true, true>
@@ -638,8 +691,8 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
                                                                                         : pairs_kernel<false, false>;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, stream, square->d_scene, square->d_bands,
                        (const qm_receiver*)d_receivers.p, square->d_sources_by_input, (const uint32_t*)d_pp.p,
-                       (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr), (uint32_t)n_pairs,
-                       (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
+                       (const uint32_t*)d_ps.p, (const float*)(pair_weight ? d_pw.p : nullptr),
+                       (const uint32_t*)d_order.p, (uint32_t)n_pairs, (float*)d_sums.p, (uint32_t*)d_failed.p, (float*)(point_bands ? d_bands.p : nullptr),
                        (float*)(pair_bands ? d_pair_bands.p : nullptr));
     QM_TRY(hipGetLastError());
     QM_TRY(hipMemcpyAsync(sums, d_sums.p, sizeof(float) * n_pairs * QM_PAIR_SUMS, hipMemcpyDeviceToHost, stream));

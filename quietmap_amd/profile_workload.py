@@ -16,6 +16,10 @@ loudness kernel (pixel_loudness_kernel) over `--pixels` synthetic packed pixels,
 `--mosaic-batch` tiles (tiles_kernel), and
 `phases` paints the `paint` window through the phase-clock build (libqmpaint_phases.so, alone in
 its process) and prints how the waves' time splits over the phases of a path, and
+`pairs` evaluates 2,000,000 point pairs of the tile-scale scene through Gpu::evaluate_pairs
+(pair_order_kernel, the radix sort and pairs_kernel): the sources nearest the centre of its
+256 x 256 central receivers, all of each source's receivers within 2 km in the caller's row-major
+order (a candidate's pairs as the Batch hands them over), then the same pairs shuffled, and
 `copy` makes one device-to-device copy of `--copy-mib` MiB, whose
 FETCH_SIZE and WRITE_SIZE calibrate the counters against a known byte count. Each step prints a
 line when it ends.
@@ -143,6 +147,28 @@ def main():
         nd = gpu.pixel_loudness(packed, offsets)
         print(f"pixels: {args.pixels} pixels, place Nden mean {float(nd[:, 6].mean()):.2f} sone, "
               f"{time.time() - t:.2f} s", flush=True)
+    if "pairs" in kernels:
+        sc = scenes.tile_scene(scale=args.scale)
+        points, nx, ny = sc.window(256)
+        square = gpu.upload(sc.scene, sc.sources)
+        px, py = points["x_m"].astype(np.float64), points["y_m"].astype(np.float64)
+        sx, sy = sc.sources["x_m"], sc.sources["y_m"]
+        order = np.argsort((sx - px.mean()) ** 2 + (sy - py.mean()) ** 2, kind="stable")
+        pp, ps, total = [], [], 0
+        for s in order:
+            inside = np.flatnonzero((px - sx[s]) ** 2 + (py - sy[s]) ** 2 <= sc.cutoff_m ** 2)
+            pp.append(inside.astype(np.uint32))
+            ps.append(np.full(inside.size, s, np.uint32))
+            total += inside.size
+            if total >= 2_000_000:
+                break
+        pp, ps = np.concatenate(pp)[:2_000_000], np.concatenate(ps)[:2_000_000]
+        shuffle = np.random.default_rng(5).permutation(pp.size)
+        for name, a, b in (("candidate order", pp, ps), ("shuffled", pp[shuffle], ps[shuffle])):
+            t = time.time()
+            r = gpu.evaluate_pairs(square, points, a, b, point_bands=False)
+            print(f"pairs ({name}): {a.size} pairs, {int((r['failed'] != 0).sum())} failed, "
+                  f"{time.time() - t:.2f} s with transfers", flush=True)
     if "flights" in kernels:
         from . import flights
 
