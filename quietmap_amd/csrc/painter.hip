@@ -632,8 +632,11 @@ int qm_evaluate_pairs(qm_gpu* gpu, const qm_square* square, const qm_point* poin
             return fail(err, errlen, "pair %zu: source %u out of range", i, pair_source[i]);
     }
     if (check_device(gpu, err, errlen)) return 1;
-    if (point_bands) std::memset(point_bands, 0, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS);
-    if (n_pairs == 0) return 0;
+    // With pairs, the point bands come back whole from the card (cleared there); without, zeros.
+    if (n_pairs == 0) {
+        if (point_bands) std::memset(point_bands, 0, sizeof(float) * n_points * QM_LAYERS * PERIODS * BANDS);
+        return 0;
+    }
     hipStream_t stream;
     QM_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     struct StreamGuard {
