@@ -97,12 +97,11 @@ struct AircraftArgs {
 __device__ __forceinline__ void position_f(const Tables& T, float slant, int& interval, float& fraction, float& s) {
     s = fmaxf(slant, (float)NPD_NEAREST_SLANT_M);
     const float log_d = log10_f(s * (float)(1.0 / METRES_PER_FOOT));
+    // The last NPD distance below the position (0 when none): the distances rise, so it is the
+    // count of them passed, without a divergent search.
     interval = 0;
-    for (int k = ND - 2; k >= 1; --k)
-        if (log_d > T.log_npd[k]) {
-            interval = k;
-            break;
-        }
+#pragma unroll
+    for (int k = 1; k <= ND - 2; ++k) interval += log_d > T.log_npd[k] ? 1 : 0;
     fraction = div_f(log_d - T.log_npd[interval], T.log_npd[interval + 1] - T.log_npd[interval]);
 }
 

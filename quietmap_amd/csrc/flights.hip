@@ -122,14 +122,13 @@ __device__ void piece_sel_f(const FlightArgs& g, const qm_flight_box& b, const d
                                     cp.on_line[2] * cp.on_line[2]);
     // NpdPosition::at (npd.rs:121-160).
     const float s = fmaxf(slant, (float)NPD_NEAREST_SLANT_M);
-    const float log_d = log10f(s / (float)METRES_PER_FOOT);
+    const float log_d = log10_f(s * (float)(1.0 / METRES_PER_FOOT));
+    // The last NPD distance below the position (0 when none), as a count of the rising distances
+    // passed.
     int interval = 0;
-    for (int k = ND - 2; k >= 1; --k)
-        if (log_d > g.log_npd[k]) {
-            interval = k;
-            break;
-        }
-    const float fraction = (log_d - g.log_npd[interval]) / (g.log_npd[interval + 1] - g.log_npd[interval]);
+#pragma unroll
+    for (int k = 1; k <= ND - 2; ++k) interval += log_d > g.log_npd[k] ? 1 : 0;
+    const float fraction = div_f(log_d - g.log_npd[interval], g.log_npd[interval + 1] - g.log_npd[interval]);
     const float lg_scaled = b.lg_scaled_distance[interval] +
                             fraction * (b.lg_scaled_distance[interval + 1] - b.lg_scaled_distance[interval]);
     const float scaled = exp10f(lg_scaled);
